@@ -1,0 +1,117 @@
+/*
+ * isar_imaging_ext.h — plain C ABI of the MI355X imaging pipeline (plain pointers and sizes).
+ *
+ * This header sits beside the drop-in isar_imaging_export.h. It exposes the same imaging chain
+ * (isar_main.cu:3-227 of the reference) as batched, device-resident entry points for callers
+ * that bring their own frames (synthetic or ingested), plus the per-frame decisions for
+ * verification. All functions return 0 on success and a negative code on failure; the reason
+ * is available from isar_last_error(). Nothing here terminates the process.
+ */
+#ifndef ISAR_IMAGING_EXT_H_
+#define ISAR_IMAGING_EXT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(_WIN32)
+#define ISAR_API __declspec(dllexport)
+#else
+#define ISAR_API __attribute__((visibility("default")))
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum isar_status {
+    ISAR_OK = 0,
+    ISAR_ERR_INVALID = -1,      /* bad argument or unsupported size                */
+    ISAR_ERR_DEVICE = -2,       /* HIP runtime error                               */
+    ISAR_ERR_MEMORY = -3,       /* device allocation failed                        */
+    ISAR_ERR_STATE = -4,        /* call out of order (e.g. run before init)        */
+    ISAR_ERR_IO = -5            /* file not found / unreadable                     */
+};
+
+/* RadarParameters (common.cuh:96-117). range_num_cut must be 512. */
+typedef struct isar_radar_params {
+    int32_t echo_num;       /* M pulses; 64, 128, 192, 256, 384, 512 or 1024            */
+    int32_t range_num;      /* N range cells; 1024, 2048, 4096, 8192 or 16384           */
+    int32_t range_num_cut;  /* W = 512                                                   */
+    int32_t reserved;
+    int64_t band_width;     /* Hz */
+    int64_t fc;             /* Hz */
+    double  Tp;             /* s  */
+    int64_t Fs;             /* Hz */
+} isar_radar_params;
+
+/* Switches of ISAR_RD_Imaging_Main_Ku (isar_main.cu:3-4) plus two additions. */
+typedef struct isar_options {
+    int32_t data_type;        /* 1 IFDS, 2 STRETCH                                            */
+    int32_t option_alignment; /* accepted and ignored, as in the reference                    */
+    int32_t option_phase;     /* accepted and ignored, as in the reference                    */
+    int32_t if_hrrp;
+    int32_t if_hpc;
+    int32_t if_mtrc;
+    int32_t quirk_q1;         /* 1 (default): slow-time window as isar_main.cu:192 applies it  */
+    int32_t iterations;       /* fast-entropy iterations (reference: 120)                      */
+    int32_t warm_start;       /* 1: phi0 supplied by the caller seeds the autofocus            */
+    int32_t reserved;
+} isar_options;
+
+/* Per-frame decisions and vectors copied back by isar_get_trace. Pointers are caller-owned
+ * and may be NULL. */
+typedef struct isar_frame_trace {
+    int32_t n_pairs;          /* M - 1 */
+    int32_t center_applied;
+    int32_t center_shift;
+    int32_t idx1_len;
+    int32_t idx2_len;
+    int32_t diff_imin;
+    int32_t cut_offset;
+    int32_t tgt_num;
+    int32_t unit2_num;
+    int32_t status;
+    float   center_thr;
+    float   pa_threshold;
+    float*  ra_lag;           /* [M-1] level-major, pair-minor */
+    int32_t* ra_imax;         /* [M-1] */
+    float*  pa_colmax;        /* [W] */
+    float*  pa_colE;          /* [W] */
+    float*  pa_colH;          /* [W] */
+    int32_t* select_bin;      /* [W], first unit2_num valid */
+    float*  phi0;             /* [2M] interleaved */
+    float*  phi;              /* [2M] interleaved */
+} isar_frame_trace;
+
+typedef struct isar_context isar_context;
+
+ISAR_API void isar_default_options(isar_options* opt);
+ISAR_API int isar_create(const isar_radar_params* p, int max_frames, isar_context** ctx);
+ISAR_API int isar_destroy(isar_context* ctx);
+ISAR_API size_t isar_workspace_bytes(const isar_radar_params* p, int max_frames);
+
+/* Device-resident batch: echo K*M*N complex float, velocity K*M double, phi0 K*M complex
+ * float (only with warm_start), img K*M*W float. Asynchronous on `stream` (hipStream_t,
+ * NULL = default stream). No device memory is allocated. */
+ISAR_API int isar_run_device(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity,
+                             const void* d_phi0, float* d_img, const isar_options* opt, void* stream);
+
+/* Host batch: copies in, runs, copies the images out; synchronous. */
+ISAR_API int isar_run_host(isar_context* ctx, int frames, const float* h_echo, const double* h_velocity,
+                           const float* h_phi0, float* h_img, const isar_options* opt);
+
+/* After a run has completed: decisions of frame `frame` of the last batch. */
+ISAR_API int isar_get_trace(isar_context* ctx, int frame, isar_frame_trace* out);
+
+/* Unnormalised n-point transforms of `batch` contiguous rows on the device (dir -1 forward,
+ * +1 inverse); n in {64,128,192,256,384,512,1024,2048,4096,8192,16384}. Synchronous. */
+ISAR_API int isar_fft_host(int n, int dir, const float* h_in, float* h_out, int batch);
+
+ISAR_API const char* isar_last_error(void);
+ISAR_API int isar_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ISAR_IMAGING_EXT_H_ */

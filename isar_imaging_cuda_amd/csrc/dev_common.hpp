@@ -1,0 +1,483 @@
+// dev_common.hpp — device primitives for the gfx950 imaging kernels: complex arithmetic,
+// wave64/workgroup reductions, exact phase generation and an LDS-resident Stockham FFT.
+#pragma once
+
+#include "pipeline_types.hpp"
+
+namespace isar {
+
+// ------------------------------------------------------------------------------------------
+// complex float helpers (float2 = re, im)
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// conj(a) * b
+__device__ __forceinline__ float2 cconjmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
+__device__ __forceinline__ float cabsf_(float2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
+// multiply by (dir * i): dir = -1 -> -i, dir = +1 -> +i
+template <int DIR>
+__device__ __forceinline__ float2 cmul_i(float2 a) {
+    return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// exp(j pi t) for a phase given in half-turns; t is reduced in FP64 before the FP32 sincospi,
+// so arguments carrying n^2 or k*s growth keep their digits.
+__device__ __forceinline__ float2 expjpi_reduced(double t) {
+    t -= 2.0 * floor(t * 0.5);                     // [0, 2)
+    if (t > 1.0) t -= 2.0;                         // (-1, 1]
+    float s, c;
+    sincospif(static_cast<float>(t), &s, &c);
+    return make_float2(c, s);
+}
+// exp(j pi t) for |t| small enough that FP32 holds it exactly (dyadic FFT twiddles).
+__device__ __forceinline__ float2 expjpi(float t) {
+    float s, c;
+    sincospif(t, &s, &c);
+    return make_float2(c, s);
+}
+
+// ------------------------------------------------------------------------------------------
+// reductions
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ float2 wave_sum2(float2 v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v.x += __shfl_xor(v.x, o, 64);
+        v.y += __shfl_xor(v.y, o, 64);
+    }
+    return v;
+}
+// (value, index) argmax with lowest index on ties
+__device__ __forceinline__ void wave_argmax(float& v, int& i) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        float ov = __shfl_xor(v, o, 64);
+        int oi = __shfl_xor(i, o, 64);
+        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+}
+// (value, index) argmin with lowest index on ties
+__device__ __forceinline__ void wave_argmin(float& v, int& i) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        float ov = __shfl_xor(v, o, 64);
+        int oi = __shfl_xor(i, o, 64);
+        if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+}
+
+// Workgroup-wide reductions through a small LDS scratch (>= 16 slots). All threads get the
+// result. Must be called by every thread of the workgroup.
+template <int NT>
+__device__ __forceinline__ float block_sum(float v, float* red) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    float r = 0.f;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) r += red[i];
+    return r;
+}
+template <int NT>
+__device__ __forceinline__ float block_max(float v, float* red) {
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) red[w] = v;
+    __syncthreads();
+    float r = red[0];
+#pragma unroll
+    for (int i = 1; i < NT / 64; ++i) r = fmaxf(r, red[i]);
+    return r;
+}
+template <int NT>
+__device__ __forceinline__ void block_argmax(float& v, int& i, float* redv, int* redi) {
+    wave_argmax(v, i);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) { redv[w] = v; redi[w] = i; }
+    __syncthreads();
+    v = redv[0];
+    i = redi[0];
+#pragma unroll
+    for (int k = 1; k < NT / 64; ++k) {
+        float ov = redv[k];
+        int oi = redi[k];
+        if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+}
+template <int NT>
+__device__ __forceinline__ void block_argmin(float& v, int& i, float* redv, int* redi) {
+    wave_argmin(v, i);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) { redv[w] = v; redi[w] = i; }
+    __syncthreads();
+    v = redv[0];
+    i = redi[0];
+#pragma unroll
+    for (int k = 1; k < NT / 64; ++k) {
+        float ov = redv[k];
+        int oi = redi[k];
+        if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
+    }
+}
+// Exclusive prefix sum of ints over the workgroup; returns the exclusive prefix, total in *tot.
+template <int NT>
+__device__ __forceinline__ int block_exclusive_scan(int v, int* red, int* tot) {
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int y = __shfl_up(x, o, 64);
+        if (l >= o) x += y;
+    }
+    __syncthreads();
+    if (l == 63) red[w] = x;
+    __syncthreads();
+    int base = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < NT / 64; ++k) {
+        int r = red[k];
+        if (k < w) base += r;
+        all += r;
+    }
+    *tot = all;
+    return base + x - v;
+}
+
+// ------------------------------------------------------------------------------------------
+// Small DFTs in registers. DIR = -1 forward (exp(-j...)), +1 inverse. Output in natural order.
+// ------------------------------------------------------------------------------------------
+template <int DIR>
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+    float2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+template <int DIR>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+    float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+    float2 t2 = cadd(a1, a3), t3 = cmul_i<DIR>(csub(a1, a3));
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
+}
+template <int DIR>
+__device__ __forceinline__ void dft3(float2& a0, float2& a1, float2& a2) {
+    constexpr float h = 0.86602540378443864676f;   // sqrt(3)/2
+    float2 t = cadd(a1, a2), d = csub(a1, a2);
+    float2 m = make_float2(a0.x - 0.5f * t.x, a0.y - 0.5f * t.y);
+    float2 r = cmul_i<DIR>(make_float2(h * d.x, h * d.y));
+    a0 = cadd(a0, t);
+    a1 = cadd(m, r);
+    a2 = csub(m, r);
+}
+
+template <int R, int DIR>
+struct Dft;
+
+template <int DIR>
+struct Dft<2, DIR> {
+    __device__ __forceinline__ static void run(float2* v) { dft2<DIR>(v[0], v[1]); }
+};
+template <int DIR>
+struct Dft<3, DIR> {
+    __device__ __forceinline__ static void run(float2* v) { dft3<DIR>(v[0], v[1], v[2]); }
+};
+template <int DIR>
+struct Dft<4, DIR> {
+    __device__ __forceinline__ static void run(float2* v) { dft4<DIR>(v[0], v[1], v[2], v[3]); }
+};
+template <int DIR>
+struct Dft<8, DIR> {
+    // 8 = 4 (inner, over n1 with stride 2) x 2
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float r = 0.70710678118654752440f;
+        float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+        float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+        dft4<DIR>(e0, e1, e2, e3);
+        dft4<DIR>(o0, o1, o2, o3);
+        // twiddle o_k *= W8^k, W8 = exp(DIR j pi/4)
+        o1 = cmul(o1, make_float2(r, DIR * r));
+        o2 = cmul_i<DIR>(o2);
+        o3 = cmul(o3, make_float2(-r, DIR * r));
+        v[0] = cadd(e0, o0); v[4] = csub(e0, o0);
+        v[1] = cadd(e1, o1); v[5] = csub(e1, o1);
+        v[2] = cadd(e2, o2); v[6] = csub(e2, o2);
+        v[3] = cadd(e3, o3); v[7] = csub(e3, o3);
+    }
+};
+template <int DIR>
+struct Dft<16, DIR> {
+    // 16 = 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2
+    __device__ __forceinline__ static void run(float2* v) {
+        constexpr float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+        constexpr float r = 0.70710678118654752440f;
+        float2 b[4][4];
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) {
+            b[n2][0] = v[n2]; b[n2][1] = v[4 + n2]; b[n2][2] = v[8 + n2]; b[n2][3] = v[12 + n2];
+            dft4<DIR>(b[n2][0], b[n2][1], b[n2][2], b[n2][3]);
+        }
+        // twiddles W16^(n2 k1)
+        b[1][1] = cmul(b[1][1], make_float2(c1, DIR * s1));
+        b[1][2] = cmul(b[1][2], make_float2(r, DIR * r));
+        b[1][3] = cmul(b[1][3], make_float2(s1, DIR * c1));
+        b[2][1] = cmul(b[2][1], make_float2(r, DIR * r));
+        b[2][2] = cmul_i<DIR>(b[2][2]);
+        b[2][3] = cmul(b[2][3], make_float2(-r, DIR * r));
+        b[3][1] = cmul(b[3][1], make_float2(s1, DIR * c1));
+        b[3][2] = cmul(b[3][2], make_float2(-r, DIR * r));
+        b[3][3] = cmul(b[3][3], make_float2(-c1, -DIR * s1));
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) {
+            float2 a0 = b[0][k1], a1 = b[1][k1], a2 = b[2][k1], a3 = b[3][k1];
+            dft4<DIR>(a0, a1, a2, a3);
+            v[k1] = a0; v[k1 + 4] = a1; v[k1 + 8] = a2; v[k1 + 12] = a3;
+        }
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// LDS-resident Stockham FFT (radix-2/3/4/8/16 passes).
+//
+// A transform of length N is computed by T = N / E threads, each holding E elements in
+// registers. Pass p with radix R processes N/R butterflies; thread t owns butterflies
+// j = t + b*T, b < E/R, and holds x[j + r*N/R] for r < R on entry. After the last pass the
+// thread holds the outputs at the same positions j + r*N/R_last, so the first-pass load and
+// the last-pass store are both unit-stride across threads (coalesced against global memory).
+// Between passes the data goes through an LDS buffer with one padding slot every 16 elements.
+// All threads of the workgroup must call the transform together (it uses __syncthreads).
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int lds_padded_size(int n) { return n + (n >> 4) + 1; }
+
+template <int N, int E, int R, int DIR, int Ns>
+__device__ __forceinline__ void stockham_compute(float2 (&x)[E], int tid) {
+    constexpr int T = N / E;
+    constexpr int B = E / R;
+    static_assert(E % R == 0, "radix must divide the elements per thread");
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = tid + b * T;
+        float2* v = &x[b * R];
+        if constexpr (Ns > 1) {
+            const int k = j % Ns;
+            // twiddles exp(DIR j 2 pi k r / (Ns R)); 2 k r / (Ns R) is exact in FP32. Even
+            // powers up to 8 come straight from sincospi, the rest are one or two products.
+            const float base = static_cast<float>(2 * k) / static_cast<float>(Ns * R);
+            float2 w[R];
+            w[0] = make_float2(1.f, 0.f);
+            w[1] = expjpi(DIR * base);
+#pragma unroll
+            for (int r = 2; r < R; ++r) {
+                if ((r & 1) == 0 && r <= 8) w[r] = expjpi(DIR * base * static_cast<float>(r));
+                else if (r < 8) w[r] = cmul(w[r - 1], w[1]);
+                else w[r] = cmul(w[8], w[r - 8]);
+            }
+#pragma unroll
+            for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
+        }
+        Dft<R, DIR>::run(v);
+    }
+}
+
+template <int N, int E, int R, int Ns>
+__device__ __forceinline__ void stockham_store(const float2 (&x)[E], float2* lds, int tid) {
+    constexpr int T = N / E;
+    constexpr int B = E / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = tid + b * T;
+        const int k = j % Ns;
+        const int d = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[lds_pad(d + r * Ns)] = x[b * R + r];
+    }
+}
+
+template <int N, int E, int R>
+__device__ __forceinline__ void stockham_load(float2 (&x)[E], const float2* lds, int tid) {
+    constexpr int T = N / E;
+    constexpr int B = E / R;
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+        const int j = tid + b * T;
+#pragma unroll
+        for (int r = 0; r < R; ++r) x[b * R + r] = lds[lds_pad(j + r * (N / R))];
+    }
+}
+
+// Element position held in register slot e after the transform (for the last radix RL).
+template <int N, int E, int RL>
+__device__ __forceinline__ int fft_out_pos(int tid, int e) {
+    constexpr int T = N / E;
+    return (tid + (e / RL) * T) + (e % RL) * (N / RL);
+}
+// Element position expected in register slot e before the transform (first radix R0).
+template <int N, int E, int R0>
+__device__ __forceinline__ int fft_in_pos(int tid, int e) {
+    constexpr int T = N / E;
+    return (tid + (e / R0) * T) + (e % R0) * (N / R0);
+}
+
+// Radix sequence per transform length for E elements per thread.
+template <int N, int E>
+struct FftPlan;
+
+// E = 16 plans
+template <> struct FftPlan<16384, 16> { static constexpr int NP = 4; static constexpr int R[4] = {16, 16, 16, 4}; };
+template <> struct FftPlan<8192, 16>  { static constexpr int NP = 4; static constexpr int R[4] = {16, 16, 16, 2}; };
+template <> struct FftPlan<4096, 16>  { static constexpr int NP = 3; static constexpr int R[3] = {16, 16, 16}; };
+template <> struct FftPlan<2048, 16>  { static constexpr int NP = 3; static constexpr int R[3] = {16, 16, 8}; };
+template <> struct FftPlan<1024, 16>  { static constexpr int NP = 3; static constexpr int R[3] = {16, 16, 4}; };
+template <> struct FftPlan<512, 16>   { static constexpr int NP = 3; static constexpr int R[3] = {16, 16, 2}; };
+template <> struct FftPlan<256, 16>   { static constexpr int NP = 2; static constexpr int R[2] = {16, 16}; };
+template <> struct FftPlan<128, 16>   { static constexpr int NP = 2; static constexpr int R[2] = {16, 8}; };
+template <> struct FftPlan<64, 16>    { static constexpr int NP = 2; static constexpr int R[2] = {16, 4}; };
+// E = 8 plans
+template <> struct FftPlan<512, 8>    { static constexpr int NP = 3; static constexpr int R[3] = {8, 8, 8}; };
+template <> struct FftPlan<4096, 8>   { static constexpr int NP = 4; static constexpr int R[4] = {8, 8, 8, 8}; };
+// E = 12 plans (mixed radix for 3 * 2^k lengths)
+template <> struct FftPlan<192, 12>   { static constexpr int NP = 4; static constexpr int R[4] = {4, 4, 4, 3}; };
+template <> struct FftPlan<384, 12>   { static constexpr int NP = 5; static constexpr int R[5] = {4, 4, 4, 2, 3}; };
+template <> struct FftPlan<768, 12>   { static constexpr int NP = 5; static constexpr int R[5] = {4, 4, 4, 4, 3}; };
+
+template <int N, int E, int DIR, int P, int Ns>
+struct FftPasses {
+    using Plan = FftPlan<N, E>;
+    // runs passes P..NP-1; x holds the layout of pass P on entry
+    __device__ __forceinline__ static void run(float2 (&x)[E], float2* lds, int tid) {
+        constexpr int R = Plan::R[P];
+        stockham_compute<N, E, R, DIR, Ns>(x, tid);
+        if constexpr (P + 1 < Plan::NP) {
+            constexpr int RN = Plan::R[P + 1];
+            stockham_store<N, E, R, Ns>(x, lds, tid);
+            __syncthreads();
+            stockham_load<N, E, RN>(x, lds, tid);
+            __syncthreads();
+            FftPasses<N, E, DIR, P + 1, Ns * R>::run(x, lds, tid);
+        }
+    }
+};
+
+// In-register/LDS transform. On entry x[e] = in[fft_in_pos(tid, e)], on exit
+// x[e] = out[fft_out_pos(tid, e)]. lds must hold lds_padded_size(N) float2 for this transform.
+template <int N, int E, int DIR>
+__device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid) {
+    FftPasses<N, E, DIR, 0, 1>::run(x, lds, tid);
+}
+
+template <int N, int E>
+__device__ __forceinline__ constexpr int fft_first_radix() { return FftPlan<N, E>::R[0]; }
+template <int N, int E>
+__device__ __forceinline__ constexpr int fft_last_radix() { return FftPlan<N, E>::R[FftPlan<N, E>::NP - 1]; }
+
+// Hamming window 0.54 - 0.46 cos(2 pi n / (len - 1))  (genHammingVec, common.cu:260-266)
+__device__ __forceinline__ float hamming(int n, int len) {
+    return 0.54f - 0.46f * cospif(2.0f * static_cast<float>(n) / static_cast<float>(len - 1));
+}
+
+}  // namespace isar
+
+namespace isar {
+
+// Sum over aligned groups of WIDTH lanes (WIDTH a power of two <= 64).
+template <int WIDTH>
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+    for (int o = WIDTH / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WIDTH);
+    return v;
+}
+template <int WIDTH>
+__device__ __forceinline__ float2 group_sum2(float2 v) {
+#pragma unroll
+    for (int o = WIDTH / 2; o > 0; o >>= 1) {
+        v.x += __shfl_xor(v.x, o, WIDTH);
+        v.y += __shfl_xor(v.y, o, WIDTH);
+    }
+    return v;
+}
+
+// Inclusive prefix sum in FP64 of n values held as v[i] (i < n) in LDS, in place, by the
+// whole workgroup (NT threads). Each thread scans a contiguous segment sequentially.
+template <int NT>
+__device__ __forceinline__ void block_inclusive_scan_d(double* v, int n, double* red) {
+    const int tid = threadIdx.x;
+    const int seg = (n + NT - 1) / NT;
+    const int b = tid * seg, e = min(b + seg, n);
+    double run = 0.0;
+    for (int i = b; i < e; ++i) { run += v[i]; v[i] = run; }
+    __syncthreads();
+    red[tid] = run;
+    __syncthreads();
+    if (tid == 0) {
+        double acc = 0.0;
+        for (int i = 0; i < NT; ++i) { const double t = red[i]; red[i] = acc; acc += t; }
+    }
+    __syncthreads();
+    const double base = red[tid];
+    for (int i = b; i < e; ++i) v[i] += base;
+    __syncthreads();
+}
+
+// Column-transform configuration for M-point slow-time transforms inside a 256-thread block.
+template <int M>
+struct ColCfg {
+    static constexpr int E = (M == 192 || M == 384 || M == 768) ? 12 : 16;
+    static constexpr int TPF = M / E;                  // threads per transform
+    static constexpr int NT = 256;
+    static constexpr int CT = NT / TPF;                // transforms (columns) per block
+    static constexpr int R0 = FftPlan<M, E>::R[0];
+    static constexpr int RL = FftPlan<M, E>::R[FftPlan<M, E>::NP - 1];
+    static constexpr bool same_layout = (R0 == RL);
+    __device__ static int in(int lane, int e) { return fft_in_pos<M, E, R0>(lane, e); }
+    __device__ static int out(int lane, int e) { return fft_out_pos<M, E, RL>(lane, e); }
+};
+
+// Move a transform result from the last-pass layout to the first-pass layout through LDS.
+template <int M, int E>
+__device__ __forceinline__ void relayout_out_to_in(float2 (&x)[E], float2* buf, int lane) {
+    constexpr int R0 = FftPlan<M, E>::R[0];
+    constexpr int RL = FftPlan<M, E>::R[FftPlan<M, E>::NP - 1];
+    if constexpr (R0 != RL) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) buf[lds_pad(fft_out_pos<M, E, RL>(lane, e))] = x[e];
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < E; ++e) x[e] = buf[lds_pad(fft_in_pos<M, E, R0>(lane, e))];
+        __syncthreads();
+    }
+}
+
+}  // namespace isar

@@ -1,0 +1,375 @@
+// pipeline.cpp — imaging context and the batch launch sequence (isar_main.cu:3-227).
+#include "pipeline.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+namespace isar {
+
+hipError_t configure_ra_kernels();
+hipError_t configure_center_kernels();
+hipError_t configure_pa_kernels();
+hipError_t configure_mtrc_kernels();
+hipError_t configure_debug_kernels();
+
+namespace {
+std::mutex g_err_mu;
+std::string g_last_error;
+constexpr double kLightSpeed = 3e8;   // LIGHT_SPEED (common.cuh:69)
+}  // namespace
+
+void set_error(const std::string& msg) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_last_error = msg;
+}
+int fail(int code, const std::string& msg) {
+    set_error(msg);
+    return code;
+}
+const char* last_error_cstr() {
+    thread_local std::string copy;
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    copy = g_last_error;
+    return copy.c_str();
+}
+
+hipError_t configure_kernels() {
+    static std::once_flag once;
+    static hipError_t result = hipSuccess;
+    std::call_once(once, [] {
+        hipError_t errs[] = {configure_ra_kernels(), configure_center_kernels(), configure_pa_kernels(),
+                             configure_mtrc_kernels(), configure_debug_kernels()};
+        for (hipError_t e : errs)
+            if (e != hipSuccess) result = e;
+    });
+    return result;
+}
+
+int next_pow2_ref(int N) {
+    int n = 1;
+    while (N >> 1) {
+        n <<= 1;
+        N >>= 1;
+    }
+    return n << 1;
+}
+
+int validate_params(const isar_radar_params& p) {
+    // imagingMemInit (isar_imaging_export.cpp:144-155) plus the sizes the kernels exist for
+    if (p.range_num_cut != kRangeCut) return fail(ISAR_ERR_INVALID, "range_num_cut must be 512");
+    if (p.echo_num > 1024) return fail(ISAR_ERR_INVALID, "echo_num > 1024");
+    if (p.range_num < p.range_num_cut) return fail(ISAR_ERR_INVALID, "range_num < range_num_cut");
+    if (p.range_num < p.echo_num) return fail(ISAR_ERR_INVALID, "range_num < echo_num");
+    if (!pulse_count_supported(p.echo_num))
+        return fail(ISAR_ERR_INVALID, "echo_num " + std::to_string(p.echo_num) + " not supported");
+    if (!range_size_supported(p.range_num))
+        return fail(ISAR_ERR_INVALID, "range_num " + std::to_string(p.range_num) + " not supported");
+    if (!(p.band_width > 0) || !(p.fc > 0) || !(p.Tp > 0) || !(p.Fs > 0))
+        return fail(ISAR_ERR_INVALID, "radar parameters must be positive");
+    return ISAR_OK;
+}
+
+size_t workspace_bytes(const isar_radar_params& p, int K) {
+    const size_t M = p.echo_num, N = p.range_num, W = p.range_num_cut;
+    const size_t F = next_pow2_ref(2 * p.echo_num - 1);
+    const size_t nchunks = (M + kRaFinalRows - 1) / kRaFinalRows;
+    const size_t nbands = (M + kCutBandRows - 1) / kCutBandRows;
+    size_t per = 0;
+    per += M * N * sizeof(float2);              // pre / st
+    per += 2 * M * N * sizeof(float);           // egrp, epart
+    per += nchunks * N * sizeof(float) + nchunks * sizeof(float);
+    per += M * sizeof(double) + 2 * M * sizeof(float);          // lagsum, lags, imax
+    per += sizeof(FrameDecisions);
+    per += 4 * M * W * sizeof(float2);          // x, stm, czt, su
+    per += M * sizeof(float2) * 3;              // rowdop, phi0, phi
+    per += nbands * W * sizeof(float) + 3 * W * sizeof(float) + W * sizeof(int32_t) + W * sizeof(float2);
+    return per * K + W * F * sizeof(float2);
+}
+
+Context::~Context() {
+    for (void* a : allocations) (void)hipFree(a);
+}
+
+int Context::init(const isar_radar_params& params, int frames) {
+    int rc = validate_params(params);
+    if (rc) return rc;
+    if (frames < 1) return fail(ISAR_ERR_INVALID, "max_frames must be >= 1");
+    hipError_t ce = configure_kernels();
+    if (ce != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("kernel configuration: ") + hipGetErrorString(ce));
+    p = params;
+    max_frames = frames;
+    const size_t K = frames, M = p.echo_num, N = p.range_num, W = p.range_num_cut;
+    F = next_pow2_ref(2 * p.echo_num - 1);
+    const size_t nchunks = (M + kRaFinalRows - 1) / kRaFinalRows;
+    const size_t nbands = (M + kCutBandRows - 1) / kCutBandRows;
+    bool ok = true;
+    auto alloc = [&](size_t bytes) -> void* {
+        void* ptr = nullptr;
+        if (!ok) return nullptr;
+        if (hipMalloc(&ptr, bytes ? bytes : 16) != hipSuccess) { ok = false; return nullptr; }
+        allocations.push_back(ptr);
+        return ptr;
+    };
+    Job& J = tmpl;
+    J.M = static_cast<int>(M);
+    J.N = static_cast<int>(N);
+    J.W = static_cast<int>(W);
+    J.F = F;
+    J.pre = static_cast<float2*>(alloc(K * M * N * sizeof(float2)));
+    J.st = J.pre;   // each row is read completely before its aligned profile overwrites it
+    J.egrp = static_cast<float*>(alloc(K * M * N * sizeof(float)));
+    J.epart = static_cast<float*>(alloc(K * M * N * sizeof(float)));
+    J.arp_part = static_cast<float*>(alloc(K * nchunks * N * sizeof(float)));
+    J.max_part = static_cast<float*>(alloc(K * nchunks * sizeof(float)));
+    J.lagsum = static_cast<double*>(alloc(K * M * sizeof(double)));
+    J.lags = static_cast<float*>(alloc(K * M * sizeof(float)));
+    J.imax = static_cast<int32_t*>(alloc(K * M * sizeof(int32_t)));
+    J.dec = static_cast<FrameDecisions*>(alloc(K * sizeof(FrameDecisions)));
+    J.x = static_cast<float2*>(alloc(K * M * W * sizeof(float2)));
+    J.rowdop = static_cast<float2*>(alloc(K * M * sizeof(float2)));
+    J.colmax_part = static_cast<float*>(alloc(K * nbands * W * sizeof(float)));
+    J.colmax = static_cast<float*>(alloc(K * W * sizeof(float)));
+    J.colE = static_cast<float*>(alloc(K * W * sizeof(float)));
+    J.colH = static_cast<float*>(alloc(K * W * sizeof(float)));
+    J.sel = static_cast<int32_t*>(alloc(K * W * sizeof(int32_t)));
+    J.su = static_cast<float2*>(alloc(K * W * M * sizeof(float2)));
+    J.phi0 = static_cast<float2*>(alloc(K * M * sizeof(float2)));
+    J.phi = static_cast<float2*>(alloc(K * M * sizeof(float2)));
+    J.stm = static_cast<float2*>(alloc(K * M * W * sizeof(float2)));
+    J.czt = static_cast<float2*>(alloc(K * M * W * sizeof(float2)));
+    J.colsum = static_cast<float2*>(alloc(K * W * sizeof(float2)));
+    czt_table = static_cast<float2*>(alloc(W * F * sizeof(float2)));
+    if (!ok) return fail(ISAR_ERR_MEMORY, "device allocation failed for " + std::to_string(frames) + " frames");
+    // constants (high_speed_compensation.cu:16-18, mtrc.cu:10-12, 47)
+    const double B = static_cast<double>(p.band_width), fc = static_cast<double>(p.fc), Fs = static_cast<double>(p.Fs);
+    J.hpc_k = -4.0 * (B / p.Tp) / kLightSpeed / (Fs * Fs);
+    J.czt_c0 = B / (2.0 * fc);
+    J.czt_posa = B / (fc * (static_cast<double>(W) - 1.0));
+    J.czt_table = czt_table;
+    launch_czt_table(J.M, J.W, F, J.czt_c0, J.czt_posa, czt_table, nullptr);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("chirp-z table: ") + hipGetErrorString(e));
+    return ISAR_OK;
+}
+
+int Context::run(int frames, const void* echo, const double* velocity, const void* phi0, float* img,
+                 const isar_options& opt, hipStream_t s) {
+    if (frames < 1 || frames > max_frames)
+        return fail(ISAR_ERR_INVALID, "frames must be in [1, " + std::to_string(max_frames) + "]");
+    if (!echo || !img) return fail(ISAR_ERR_INVALID, "null echo or image pointer");
+    if (opt.data_type != 1 && opt.data_type != 2) return fail(ISAR_ERR_INVALID, "data_type must be 1 (IFDS) or 2 (STRETCH)");
+    const bool hpc = opt.if_hpc && opt.data_type == 2;
+    if (hpc && !velocity) return fail(ISAR_ERR_INVALID, "high-speed compensation needs the velocity vector");
+    if (opt.warm_start && !phi0) return fail(ISAR_ERR_INVALID, "warm start needs phi0");
+    if (opt.iterations < 0) return fail(ISAR_ERR_INVALID, "iterations must be >= 0");
+    Job J = tmpl;
+    J.K = frames;
+    J.data_type = opt.data_type;
+    J.if_hpc = opt.if_hpc ? 1 : 0;
+    J.if_hrrp = opt.if_hrrp ? 1 : 0;
+    J.if_mtrc = opt.if_mtrc ? 1 : 0;
+    J.quirk_q1 = opt.quirk_q1 ? 1 : 0;
+    J.iterations = opt.iterations;
+    J.phi0_given = opt.warm_start ? 1 : 0;
+    J.echo = static_cast<const float2*>(echo);
+    J.velocity = velocity;
+    J.phi0_in = static_cast<const float2*>(phi0);
+    J.img = img;
+    const int M = J.M;
+
+    (void)hipGetLastError();
+    if (J.data_type == 1) launch_ifds_pre(J, s);
+    launch_ra_init(J, s);
+    int pair_base = 0;
+    for (int st = 1; st < M; st *= 2) {
+        const int ngroups = (M + st - 1) / st, npairs = ngroups / 2, unpaired = ngroups & 1;
+        launch_ra_corr(J, st, pair_base, npairs + unpaired, s);
+        const int R = std::min(st, kRaChunkRows), cpp = (st + R - 1) / R;
+        if (npairs > 0) launch_ra_upd(J, st, pair_base, npairs * cpp, s);
+        pair_base += npairs;
+    }
+    launch_ra_final(J, s);
+    launch_center(J, s);
+    launch_cut(J, s);
+    launch_pa_colfft(J, s);
+    launch_pa_select(J, s);
+    launch_autofocus(J, s);
+    if (J.if_mtrc) {
+        launch_mtrc_rows(J, s);
+        launch_mtrc_czt(J, s);
+        if (!J.quirk_q1) launch_col_window(J, s);
+        launch_mtrc_image(J, s);
+    } else {
+        launch_post_nomtrc(J, s);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    last_frames = frames;
+    return ISAR_OK;
+}
+
+int Context::trace(int frame, isar_frame_trace* out) {
+    if (!out || frame < 0 || frame >= last_frames) return fail(ISAR_ERR_INVALID, "no such frame in the last batch");
+    const Job& J = tmpl;
+    const size_t M = J.M, W = J.W;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("synchronize: ") + hipGetErrorString(e));
+    FrameDecisions d{};
+    e = hipMemcpy(&d, J.dec + frame, sizeof(d), hipMemcpyDeviceToHost);
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+        if (dst && e == hipSuccess) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    };
+    cp(out->ra_lag, J.lags + frame * (M - 1), (M - 1) * sizeof(float));
+    cp(out->ra_imax, J.imax + frame * (M - 1), (M - 1) * sizeof(int32_t));
+    cp(out->pa_colmax, J.colmax + frame * W, W * sizeof(float));
+    cp(out->pa_colE, J.colE + frame * W, W * sizeof(float));
+    cp(out->pa_colH, J.colH + frame * W, W * sizeof(float));
+    cp(out->select_bin, J.sel + frame * W, std::min<size_t>(W, std::max(d.unit2_num, 0)) * sizeof(int32_t));
+    cp(out->phi0, J.phi0 + frame * M, M * sizeof(float2));
+    cp(out->phi, J.phi + frame * M, M * sizeof(float2));
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("trace copy: ") + hipGetErrorString(e));
+    out->n_pairs = static_cast<int32_t>(M - 1);
+    out->center_applied = d.center_applied;
+    out->center_shift = d.center_shift;
+    out->idx1_len = d.idx1_len;
+    out->idx2_len = d.idx2_len;
+    out->diff_imin = d.diff_imin;
+    out->cut_offset = d.cut_offset;
+    out->tgt_num = d.tgt_num;
+    out->unit2_num = d.unit2_num;
+    out->status = d.status;
+    out->center_thr = d.center_thr;
+    out->pa_threshold = d.pa_threshold;
+    return ISAR_OK;
+}
+
+}  // namespace isar
+
+// ------------------------------------------------------------------------------------------
+// C ABI (isar_imaging_ext.h)
+// ------------------------------------------------------------------------------------------
+using isar::fail;
+
+struct isar_context : isar::Context {};
+
+extern "C" {
+
+ISAR_API void isar_default_options(isar_options* opt) {
+    if (!opt) return;
+    std::memset(opt, 0, sizeof(*opt));
+    opt->data_type = 2;
+    opt->option_alignment = 0;
+    opt->option_phase = 1;
+    opt->if_hrrp = 0;
+    opt->if_hpc = 1;
+    opt->if_mtrc = 1;
+    opt->quirk_q1 = 1;
+    opt->iterations = 120;   // FAST_ENTROPY_ITERATION_NUM (common.cuh:73)
+    opt->warm_start = 0;
+}
+
+ISAR_API size_t isar_workspace_bytes(const isar_radar_params* p, int max_frames) {
+    return p ? isar::workspace_bytes(*p, max_frames) : 0;
+}
+
+ISAR_API int isar_create(const isar_radar_params* p, int max_frames, isar_context** ctx) {
+    if (!p || !ctx) return fail(ISAR_ERR_INVALID, "null argument");
+    *ctx = nullptr;
+    auto* c = new isar_context();
+    int rc = c->init(*p, max_frames);
+    if (rc) {
+        delete c;
+        return rc;
+    }
+    *ctx = c;
+    return ISAR_OK;
+}
+
+ISAR_API int isar_destroy(isar_context* ctx) {
+    if (!ctx) return fail(ISAR_ERR_INVALID, "null context");
+    (void)hipDeviceSynchronize();
+    delete ctx;
+    return ISAR_OK;
+}
+
+ISAR_API int isar_run_device(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity,
+                             const void* d_phi0, float* d_img, const isar_options* opt, void* stream) {
+    if (!ctx || !opt) return fail(ISAR_ERR_INVALID, "null context or options");
+    return ctx->run(frames, d_echo, d_velocity, d_phi0, d_img, *opt, static_cast<hipStream_t>(stream));
+}
+
+ISAR_API int isar_run_host(isar_context* ctx, int frames, const float* h_echo, const double* h_velocity,
+                           const float* h_phi0, float* h_img, const isar_options* opt) {
+    if (!ctx || !opt || !h_echo || !h_img) return fail(ISAR_ERR_INVALID, "null argument");
+    if (frames < 1 || frames > ctx->max_frames) return fail(ISAR_ERR_INVALID, "frames out of range");
+    const size_t M = ctx->p.echo_num, N = ctx->p.range_num, W = ctx->p.range_num_cut;
+    // device staging buffers are allocated on first use and kept for the context's lifetime
+    if (!ctx->stage_dev[0]) {
+        const size_t K = ctx->max_frames;
+        const size_t bytes[4] = {K * M * N * sizeof(float2), K * M * sizeof(double), K * M * sizeof(float2),
+                                 K * M * W * sizeof(float)};
+        for (int i = 0; i < 4; ++i) {
+            void* ptr = nullptr;
+            if (hipMalloc(&ptr, bytes[i]) != hipSuccess) return fail(ISAR_ERR_MEMORY, "staging allocation failed");
+            ctx->allocations.push_back(ptr);
+            ctx->stage_dev[i] = ptr;
+        }
+    }
+    void* d_echo = ctx->stage_dev[0];
+    double* d_vel = static_cast<double*>(ctx->stage_dev[1]);
+    void* d_phi0 = ctx->stage_dev[2];
+    float* d_img = static_cast<float*>(ctx->stage_dev[3]);
+    hipError_t e = hipMemcpy(d_echo, h_echo, frames * M * N * sizeof(float2), hipMemcpyHostToDevice);
+    if (e == hipSuccess && h_velocity) e = hipMemcpy(d_vel, h_velocity, frames * M * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && h_phi0) e = hipMemcpy(d_phi0, h_phi0, frames * M * sizeof(float2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("upload: ") + hipGetErrorString(e));
+    int rc = ctx->run(frames, d_echo, h_velocity ? d_vel : nullptr, h_phi0 ? d_phi0 : nullptr, d_img, *opt, nullptr);
+    if (rc) return rc;
+    e = hipMemcpy(h_img, d_img, frames * M * W * sizeof(float), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("run/download: ") + hipGetErrorString(e));
+    return ISAR_OK;
+}
+
+ISAR_API int isar_get_trace(isar_context* ctx, int frame, isar_frame_trace* out) {
+    if (!ctx) return fail(ISAR_ERR_INVALID, "null context");
+    return ctx->trace(frame, out);
+}
+
+ISAR_API int isar_fft_host(int n, int dir, const float* h_in, float* h_out, int batch) {
+    if (!h_in || !h_out || batch < 1 || (dir != -1 && dir != 1)) return fail(ISAR_ERR_INVALID, "bad fft arguments");
+    hipError_t ce = isar::configure_kernels();
+    if (ce != hipSuccess) return fail(ISAR_ERR_DEVICE, "kernel configuration failed");
+    const size_t bytes = static_cast<size_t>(n) * batch * sizeof(float2);
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(ISAR_ERR_MEMORY, "fft buffers");
+    }
+    hipError_t e = hipMemcpy(a, h_in, bytes, hipMemcpyHostToDevice);
+    bool ok = e == hipSuccess && isar::debug_fft(n, dir, static_cast<const float2*>(a), static_cast<float2*>(b), batch, nullptr);
+    if (ok) {
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpy(h_out, b, bytes, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (!ok) return fail(ISAR_ERR_INVALID, "unsupported fft length " + std::to_string(n));
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("fft: ") + hipGetErrorString(e));
+    return ISAR_OK;
+}
+
+ISAR_API const char* isar_last_error(void) { return isar::last_error_cstr(); }
+
+ISAR_API int isar_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,201 @@
+// center.hip — HRRP centring (range_alignment.cu:371-465) and range-window cut
+// (common.cu:494-518) with every decision taken on the device (no host round trips).
+#include "../dev_common.hpp"
+#include "../launchers.hpp"
+
+namespace isar {
+
+namespace {
+constexpr int kCenterThreads = 1024;
+constexpr int kInterLength = 30;   // isar_main.cu:98
+constexpr int kArpWindow = 8;      // range_alignment.cu:434
+
+// ascending bitonic sort of n (power of two) floats in LDS by the whole workgroup
+template <int NT>
+__device__ void bitonic_sort_f(float* a, int n) {
+    for (int k = 2; k <= n; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n; i += NT) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const float x = a[i], y = a[l];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) { a[i] = y; a[l] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+}  // namespace
+
+// One workgroup per frame. LDS: arp[N], sorted[N], reduction scratch.
+__global__ __launch_bounds__(kCenterThreads) void k_center(Job J) {
+    HIP_DYNAMIC_SHARED(float, sm);
+    constexpr int NT = kCenterThreads;
+    const int f = blockIdx.x, tid = threadIdx.x;
+    const int M = J.M, N = J.N, W = J.W;
+    float* arp = sm;
+    float* srt = sm + N;
+    float* redv = srt + N;
+    int* redi = reinterpret_cast<int*>(redv + 32);
+    const int nchunks = (M + kRaFinalRows - 1) / kRaFinalRows;
+
+    // H = |S| / max|S| ; ARP = column sums / M  (range_alignment.cu:381-399, all M rows)
+    float gm = 0.0f;
+    for (int c = tid; c < nchunks; c += NT) gm = fmaxf(gm, J.max_part[static_cast<size_t>(f) * nchunks + c]);
+    gm = block_max<NT>(gm, redv);
+    const float inv = gm > 0.0f ? 1.0f / gm : 0.0f;
+    const float invM = 1.0f / static_cast<float>(M);
+    for (int n = tid; n < N; n += NT) {
+        float acc = 0.0f;
+        for (int c = 0; c < nchunks; ++c) acc += J.arp_part[(static_cast<size_t>(f) * nchunks + c) * N + n];
+        const float v = acc * inv * invM;
+        arp[n] = v;
+        srt[n] = v;
+    }
+    __syncthreads();
+    bitonic_sort_f<NT>(srt, N);
+
+    // threshold from the sorted ARP (range_alignment.cu:402-423)
+    const float amax = fabsf(srt[N - 1]);
+    const float extra = static_cast<float>(kInterLength) * amax / static_cast<float>(N);
+    float dmin = 3.4e38f;
+    int imin = 0x7fffffff;
+    for (int i = tid; i < N - kInterLength; i += NT) {
+        const float d = fabsf(srt[i + kInterLength] - srt[i] - extra);
+        if (d < dmin || (d == dmin && i < imin)) { dmin = d; imin = i; }
+    }
+    block_argmin<NT>(dmin, imin, redv, redi);
+    int status = 0;
+    if (imin < 0 || imin >= N - kInterLength) {   // every difference was NaN
+        imin = 0;
+        status |= 1;
+    }
+    const float thr = srt[imin + kInterLength / 2];
+
+    // idx_1 = find(arp > thr); windowed means (getARPMean, :524-550); idx_2 = find(mean < thr)
+    int c1 = 0, c2 = 0;
+    long long keep_sum = 0;
+    for (int n = tid; n < N; n += NT) {
+        if (arp[n] > thr) {
+            float acc = 0.0f;
+            float mean;
+            if (n - kArpWindow / 2 >= 0 && n + kArpWindow / 2 <= N - 1) {
+                for (int q = n - kArpWindow / 2; q <= n + kArpWindow / 2; ++q) acc += arp[q];
+                mean = acc / static_cast<float>(kArpWindow + 1);
+            } else if (n - kArpWindow / 2 < 0) {
+                for (int q = 0; q <= n + kArpWindow / 2; ++q) acc += arp[q];
+                mean = acc / (static_cast<float>(kArpWindow) / 2.0f + static_cast<float>(n) + 1.0f);
+            } else {
+                for (int q = n - kArpWindow / 2; q < N; ++q) acc += arp[q];
+                mean = acc / (static_cast<float>(N) - static_cast<float>(n) - static_cast<float>(kArpWindow) / 2.0f - 1.0f);
+            }
+            ++c1;
+            if (mean < thr) ++c2;
+            else keep_sum += n;
+        }
+    }
+    const int len1 = static_cast<int>(block_sum<NT>(static_cast<float>(c1), redv) + 0.5f);
+    const int len2 = static_cast<int>(block_sum<NT>(static_cast<float>(c2), redv) + 0.5f);
+    // the kept-index sum can exceed FP32 precision: reduce it as two exact halves
+    const float ks_hi = block_sum<NT>(static_cast<float>(keep_sum >> 12), redv);
+    const float ks_lo = block_sum<NT>(static_cast<float>(keep_sum & 4095), redv);
+    const long long ksum = (static_cast<long long>(ks_hi + 0.5f) << 12) + static_cast<long long>(ks_lo + 0.5f);
+    int shift = 0, applied = 0;
+    if (len2 != len1) {
+        const int mean_idx = static_cast<int>(ksum / (len1 - len2));
+        shift = -(mean_idx - N / 2);
+        applied = 1;
+    }
+
+    // cut offset: cublasIcamax over row 0 after the circular shift (|Re| + |Im|, first index)
+    const float2* row0 = J.st + static_cast<size_t>(f) * M * N;
+    float best = -1.0f;
+    int bi = 0x7fffffff;
+    for (int n = tid; n < N; n += NT) {
+        const float2 v = row0[((n - shift) % N + N) % N];
+        const float key = fabsf(v.x) + fabsf(v.y);
+        if (key > best || (key == best && n < bi)) { best = key; bi = n; }
+    }
+    block_argmax<NT>(best, bi, redv, redi);
+    int off = bi - W / 2;
+    if (off < 0) off = 0;
+    if (off > N - W) off = N - W;
+    if (tid == 0) {
+        FrameDecisions& d = J.dec[f];
+        d.center_shift = shift;
+        d.center_applied = applied;
+        d.idx1_len = len1;
+        d.idx2_len = len2;
+        d.diff_imin = imin;
+        d.cut_offset = off;
+        d.center_thr = thr;
+        d.gmax = gm;
+        d.status = status;
+    }
+}
+
+// Cut: X[m, j] = S[m, (off + j - shift) mod N] for a band of rows; column maxima of |X| per
+// band (fastEntropy :180-184) and the adjacent-pulse products sum_j conj(X[m,j]) X[m+1,j]
+// (dopplerTracking :12-20). Block = W threads, one per range bin of the window.
+template <int W>
+__global__ __launch_bounds__(W) void k_cut(Job J) {
+    HIP_DYNAMIC_SHARED(float, sm);
+    float* red = sm;   // 2 * (W/64) floats
+    const int band = blockIdx.x, f = blockIdx.y, j = threadIdx.x;
+    const int M = J.M, N = J.N;
+    const int nbands = (M + kCutBandRows - 1) / kCutBandRows;
+    const FrameDecisions d = J.dec[f];
+    const int q = (((d.cut_offset + j - d.center_shift) % N) + N) % N;
+    const float2* src = J.st + static_cast<size_t>(f) * M * N + q;
+    float2* xo = J.x + static_cast<size_t>(f) * M * W + j;
+    const int m0 = band * kCutBandRows, m1 = min(m0 + kCutBandRows, M);
+    float cmax = 0.0f;
+    float2 cur = src[static_cast<size_t>(m0) * N];
+    for (int m = m0; m < m1; ++m) {
+        xo[static_cast<size_t>(m) * W] = cur;
+        cmax = fmaxf(cmax, cabsf_(cur));
+        float2 prod = make_float2(0.0f, 0.0f);
+        float2 nxt = make_float2(0.0f, 0.0f);
+        if (m + 1 < M) {
+            nxt = src[static_cast<size_t>(m + 1) * N];
+            prod = cconjmul(cur, nxt);
+        }
+        // workgroup sum of the products for row m
+        float2 sum = wave_sum2(prod);
+        const int w = j >> 6, l = j & 63;
+        __syncthreads();
+        if (l == 0) { red[2 * w] = sum.x; red[2 * w + 1] = sum.y; }
+        __syncthreads();
+        if (j == 0 && m + 1 < M) {
+            float2 t = make_float2(0.0f, 0.0f);
+            for (int k = 0; k < W / 64; ++k) { t.x += red[2 * k]; t.y += red[2 * k + 1]; }
+            J.rowdop[static_cast<size_t>(f) * M + m] = t;
+        }
+        cur = nxt;
+    }
+    J.colmax_part[(static_cast<size_t>(f) * nbands + band) * W + j] = cmax;
+}
+
+hipError_t configure_center_kernels() {
+#ifndef ISAR_EMU
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_center), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>((2 * 16384 + 64) * sizeof(float)));
+#else
+    return hipSuccess;
+#endif
+}
+
+void launch_center(const Job& J, hipStream_t s) {
+    const size_t lds = (2 * static_cast<size_t>(J.N) + 64) * sizeof(float);
+    ISAR_LAUNCH(k_center, dim3(J.K), dim3(kCenterThreads), lds, s, J);
+}
+
+void launch_cut(const Job& J, hipStream_t s) {
+    const int nbands = (J.M + kCutBandRows - 1) / kCutBandRows;
+    if (J.W == kRangeCut)
+        ISAR_LAUNCH(k_cut<kRangeCut>, dim3(nbands, J.K), dim3(kRangeCut), 64 * sizeof(float), s, J);
+}
+
+}  // namespace isar

@@ -1,0 +1,327 @@
+// pa.hip — phase adjustment (phase_adjustment.cu:4-41, 167-346) for gfx950:
+// Doppler-centroid tracking, column energy/entropy statistics, bin selection and the
+// fast minimum-entropy autofocus with the selected bins kept resident per frame.
+#include <algorithm>
+
+#include "../dev_common.hpp"
+#include "../launchers.hpp"
+
+namespace isar {
+
+namespace {
+
+// phi[0] = 0, phi[m+1] = exp(-j theta_m), theta = inclusive scan of arg(r_m) (accumulated in
+// FP64 and reduced before the FP32 sincos). r: M-1 adjacent-pulse products (LDS or global).
+template <int NT>
+__device__ void doppler_phase(const float2* r, int M, float2* phi, double* scratch, double* red) {
+    for (int m = threadIdx.x; m < M - 1; m += NT) scratch[m] = static_cast<double>(atan2f(r[m].y, r[m].x));
+    __syncthreads();
+    block_inclusive_scan_d<NT>(scratch, M - 1, red);
+    for (int m = threadIdx.x; m < M; m += NT) {
+        if (m == 0) phi[0] = make_float2(0.0f, 0.0f);   // phase_adjustment.cu:29
+        else phi[m] = expjpi_reduced(-scratch[m - 1] * 0.31830988618379067154);
+    }
+    __syncthreads();
+}
+
+// ascending bitonic sort of (key, tag) pairs; ties ordered by tag (stable for unique tags)
+template <int NT>
+__device__ void bitonic_sort_kv(float* key, int* tag, int n) {
+    for (int k = 2; k <= n; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < n; i += NT) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const float a = key[i], b = key[l];
+                    const int ta = tag[i], tb = tag[l];
+                    const bool gt = (a > b) || (a == b && ta > tb) || (a != a && b == b);
+                    const bool up = (i & k) == 0;
+                    if (gt == up) { key[i] = b; key[l] = a; tag[i] = tb; tag[l] = ta; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+}  // namespace
+
+// Per column tile: Doppler-tracked image column FFT, energy E_col, entropy H_col and the
+// column maxima of |X| (phase_adjustment.cu:175-246).
+template <int M>
+__global__ __launch_bounds__(256) void k_pa_colfft(Job J) {
+    using C = ColCfg<M>;
+    constexpr int NT = C::NT, CT = C::CT, TPF = C::TPF, E = C::E;
+    HIP_DYNAMIC_SHARED(float2, sm);
+    float2* phi = sm;                                    // M
+    double* scr = reinterpret_cast<double*>(phi + M);    // M (scan) + NT (scan partials)
+    double* red = scr + M;
+    float2* work = reinterpret_cast<float2*>(red + NT);  // tile / transform buffers
+    const int tile = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const int W = J.W;
+    const int g = tid / TPF, lane = tid % TPF;
+    const int n0 = tile * CT, n = n0 + g;
+
+    doppler_phase<NT>(J.rowdop + static_cast<size_t>(f) * M, M, phi, scr, red);
+
+    const float2* X = J.x + static_cast<size_t>(f) * M * W;
+    for (int idx = tid; idx < M * CT; idx += NT) {
+        const int m = idx / CT, c = idx % CT;
+        work[m * (CT + 1) + c] = X[static_cast<size_t>(m) * W + n0 + c];
+    }
+    __syncthreads();
+    float2 x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int pos = C::in(lane, e);
+        x[e] = cmul(phi[pos], work[pos * (CT + 1) + g]);
+    }
+    __syncthreads();
+    float2* buf = work + g * lds_padded_size(M);
+    fft<M, E, -1>(x, buf, lane);
+    float p2[E];
+    float esum = 0.0f;
+#pragma unroll
+    for (int e = 0; e < E; ++e) { p2[e] = cabs2(x[e]); esum += p2[e]; }
+    const float Ecol = group_sum<TPF>(esum);
+    float h = 0.0f;
+    if (Ecol > 0.0f) {
+        const float inv = 1.0f / Ecol;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            const float p = p2[e] * inv;
+            if (p > 0.0f) h -= p * logf(p);
+        }
+    }
+    const float H = group_sum<TPF>(h);
+    if (lane == 0) {
+        const int nb = (M + kCutBandRows - 1) / kCutBandRows;
+        float cm = 0.0f;
+        for (int b = 0; b < nb; ++b) cm = fmaxf(cm, J.colmax_part[(static_cast<size_t>(f) * nb + b) * W + n]);
+        J.colmax[static_cast<size_t>(f) * W + n] = cm;
+        J.colE[static_cast<size_t>(f) * W + n] = Ecol;
+        J.colH[static_cast<size_t>(f) * W + n] = Ecol > 0.0f ? H : __int_as_float(0x7f800000);
+    }
+}
+
+// Bin selection (phase_adjustment.cu:186-264): int-truncated mean of the column maxima,
+// tgt_num bins above 1.48 * mean, the tgt_num most energetic (stable, descending), the
+// unit2_num = nextPow2(tgt_num / 2) of those with the lowest entropy (stable, ascending),
+// sorted by index. One workgroup of W threads per frame.
+template <int W>
+__global__ __launch_bounds__(W) void k_pa_select(Job J) {
+    HIP_DYNAMIC_SHARED(float, sm);
+    float* key = sm;                                  // W
+    int* tag = reinterpret_cast<int*>(key + W);       // W
+    int* order1 = tag + W;                            // W
+    float* red = reinterpret_cast<float*>(order1 + W);   // 32
+    int* ired = reinterpret_cast<int*>(red + 32);        // 32
+    const int f = blockIdx.x, n = threadIdx.x;
+    const float mx = J.colmax[static_cast<size_t>(f) * W + n];
+    const float E = J.colE[static_cast<size_t>(f) * W + n];
+    // thrust::reduce(..., 0, plus<int>()) truncates every maximum to int
+    int tot = 0;
+    (void)block_exclusive_scan<W>(static_cast<int>(mx), ired, &tot);
+    const float mean_val = static_cast<float>(tot) / static_cast<float>(W);
+    const float threshold = 1.48f * mean_val;
+    int tgt = 0;
+    (void)block_exclusive_scan<W>(mx > threshold ? 1 : 0, ired, &tgt);
+    int half = tgt / 2, U = 1;
+    while (half >> 1) { U <<= 1; half >>= 1; }
+    U <<= 1;   // nextPow2 (common.cu:521-529)
+    // sort 1: energy descending, index ascending on ties
+    key[n] = (E == E) ? -E : __int_as_float(0x7f800000);
+    tag[n] = n;
+    __syncthreads();
+    bitonic_sort_kv<W>(key, tag, W);
+    order1[n] = tag[n];
+    __syncthreads();
+    // sort 2: entropy ascending among the first tgt, rank ascending on ties
+    key[n] = n < tgt ? J.colH[static_cast<size_t>(f) * W + order1[n]] : __int_as_float(0x7f800000);
+    tag[n] = n < tgt ? n : W + n;
+    __syncthreads();
+    bitonic_sort_kv<W>(key, tag, W);
+    // first U of them (bin 0 fills positions beyond tgt, as std::vector growth does)
+    int bin = 0x7fffffff;
+    if (n < U) bin = (n < tgt) ? order1[tag[n]] : 0;
+    __syncthreads();
+    key[n] = static_cast<float>(bin == 0x7fffffff ? 1 << 30 : bin);
+    tag[n] = bin;
+    __syncthreads();
+    bitonic_sort_kv<W>(key, tag, W);
+    if (n < U) J.sel[static_cast<size_t>(f) * W + n] = tag[n];
+    if (n == 0) {
+        J.dec[f].tgt_num = tgt;
+        J.dec[f].unit2_num = U;
+        J.dec[f].pa_threshold = threshold;
+    }
+}
+
+// Fast minimum-entropy autofocus (phase_adjustment.cu:270-324): one workgroup per frame; the
+// selected bins are gathered once into a column-major copy, then J.iterations fixed-point
+// updates phi_m <- unit(sum_u conj(S[m,u]) IFFT(G |G|)[m,u] / M), G = FFT(diag(phi) S).
+template <int M>
+struct AfCfg {
+    static constexpr int E = ColCfg<M>::E;
+    static constexpr int TPF = ColCfg<M>::TPF;
+    static constexpr int NT = M <= 256 ? 256 : 512;
+    static constexpr int CF = NT / TPF;          // columns per chunk
+    static constexpr int RPT = (M + NT - 1) / NT;   // accumulator rows per thread
+};
+
+template <int M>
+__global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
+    using A = AfCfg<M>;
+    using C = ColCfg<M>;
+    constexpr int NT = A::NT, E = A::E, TPF = A::TPF, CF = A::CF, RPT = A::RPT;
+    HIP_DYNAMIC_SHARED(float2, sm);
+    float2* phi = sm;                                     // M
+    float2* rdop = phi + M;                               // M
+    double* scr = reinterpret_cast<double*>(rdop + M);    // M
+    double* red = scr + M;                                // NT
+    float2* bufs = reinterpret_cast<float2*>(red + NT);   // CF * lds_padded_size(M)
+    const int f = blockIdx.x, tid = threadIdx.x;
+    const int W = J.W;
+    const int U = J.dec[f].unit2_num;
+    const int* sel = J.sel + static_cast<size_t>(f) * W;
+    float2* su = J.su + static_cast<size_t>(f) * W * M;
+    const float2* X = J.x + static_cast<size_t>(f) * M * W;
+
+    // gather the selected bins, column-major (selectRangeBins, :335-346)
+    for (int idx = tid; idx < U * M; idx += NT) {
+        const int m = idx / U, u = idx % U;
+        su[static_cast<size_t>(u) * M + m] = X[static_cast<size_t>(m) * W + sel[u]];
+    }
+    __syncthreads();
+    if (J.phi0_given) {
+        for (int m = tid; m < M; m += NT) phi[m] = J.phi0_in[static_cast<size_t>(f) * M + m];
+        __syncthreads();
+    } else {
+        // Doppler tracking of the selected bins without compensation (:282)
+        for (int m = tid; m < M - 1; m += NT) {
+            float2 acc = make_float2(0.0f, 0.0f);
+            for (int u = 0; u < U; ++u)
+                acc = cadd(acc, cconjmul(su[static_cast<size_t>(u) * M + m], su[static_cast<size_t>(u) * M + m + 1]));
+            rdop[m] = acc;
+        }
+        __syncthreads();
+        doppler_phase<NT>(rdop, M, phi, scr, red);
+    }
+    for (int m = tid; m < M; m += NT) J.phi0[static_cast<size_t>(f) * M + m] = phi[m];
+
+    const int g = tid / TPF, lane = tid % TPF;
+    float2* buf = bufs + g * lds_padded_size(M);
+    const float invM = 1.0f / static_cast<float>(M);
+    for (int it = 0; it < J.iterations; ++it) {
+        float2 acc[RPT];   // rows tid + i * NT
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) acc[i] = make_float2(0.0f, 0.0f);
+        for (int c0 = 0; c0 < U; c0 += CF) {
+            const int u = c0 + g;
+            const bool valid = u < U;
+            const float2* col = su + static_cast<size_t>(valid ? u : 0) * M;
+            float2 s[E], x[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int pos = C::in(lane, e);
+                s[e] = valid ? col[pos] : make_float2(0.0f, 0.0f);
+                x[e] = cmul(phi[pos], s[e]);
+            }
+            fft<M, E, -1>(x, buf, lane);
+#pragma unroll
+            for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabsf_(x[e]));
+            relayout_out_to_in<M, E>(x, buf, lane);
+            fft<M, E, +1>(x, buf, lane);
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int pos = C::out(lane, e);
+                const float2 sv = C::same_layout ? s[e] : (valid ? col[pos] : make_float2(0.0f, 0.0f));
+                buf[pos] = cscale(cconjmul(sv, x[e]), invM);
+            }
+            __syncthreads();
+            const int ncol = min(CF, U - c0);
+#pragma unroll
+            for (int i = 0; i < RPT; ++i) {
+                const int m = tid + i * NT;
+                if (m < M) {
+                    float2 t = make_float2(0.0f, 0.0f);
+                    for (int k = 0; k < ncol; ++k) t = cadd(t, bufs[k * lds_padded_size(M) + m]);
+                    acc[i] = cadd(acc[i], t);
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+            const int m = tid + i * NT;
+            if (m < M) {
+                const float a = cabsf_(acc[i]);
+                phi[m] = a > 0.0f ? cscale(acc[i], 1.0f / a) : make_float2(0.0f, 0.0f);
+            }
+        }
+        __syncthreads();
+    }
+    for (int m = tid; m < M; m += NT) J.phi[static_cast<size_t>(f) * M + m] = phi[m];
+}
+
+// ------------------------------------------------------------------------------------------
+namespace {
+template <int M>
+size_t colfft_lds() {
+    using C = ColCfg<M>;
+    const size_t work = std::max(static_cast<size_t>(M) * (C::CT + 1), static_cast<size_t>(C::CT) * lds_padded_size(M));
+    return M * sizeof(float2) + (M + C::NT) * sizeof(double) + work * sizeof(float2);
+}
+template <int M>
+size_t autofocus_lds() {
+    using A = AfCfg<M>;
+    return 2 * M * sizeof(float2) + (M + A::NT) * sizeof(double) + static_cast<size_t>(A::CF) * lds_padded_size(M) * sizeof(float2);
+}
+size_t select_lds(int W) { return 3 * W * sizeof(int) + 64 * sizeof(float); }
+}  // namespace
+
+#define ISAR_PULSE_SWITCH(M, BODY)                       \
+    switch (M) {                                         \
+    case 64: { constexpr int MM = 64; BODY; } break;     \
+    case 128: { constexpr int MM = 128; BODY; } break;   \
+    case 192: { constexpr int MM = 192; BODY; } break;   \
+    case 256: { constexpr int MM = 256; BODY; } break;   \
+    case 384: { constexpr int MM = 384; BODY; } break;   \
+    case 512: { constexpr int MM = 512; BODY; } break;   \
+    case 1024: { constexpr int MM = 1024; BODY; } break; \
+    default: break;                                      \
+    }
+
+bool pulse_count_supported(int M) {
+    return M == 64 || M == 128 || M == 192 || M == 256 || M == 384 || M == 512 || M == 1024;
+}
+
+hipError_t configure_pa_kernels() {
+    hipError_t err = hipSuccess;
+#ifndef ISAR_EMU
+    auto set = [&](const void* fn, size_t bytes) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
+        if (e != hipSuccess) err = e;
+    };
+#define ISAR_SET_PA(MM)                                                          \
+    set(reinterpret_cast<const void*>(&k_pa_colfft<MM>), colfft_lds<MM>());      \
+    set(reinterpret_cast<const void*>(&k_autofocus<MM>), autofocus_lds<MM>());
+    ISAR_SET_PA(64) ISAR_SET_PA(128) ISAR_SET_PA(192) ISAR_SET_PA(256) ISAR_SET_PA(384) ISAR_SET_PA(512)
+    ISAR_SET_PA(1024)
+#undef ISAR_SET_PA
+#endif
+    return err;
+}
+
+void launch_pa_colfft(const Job& J, hipStream_t s) {
+    ISAR_PULSE_SWITCH(J.M, ISAR_LAUNCH(k_pa_colfft<MM>, dim3(J.W / ColCfg<MM>::CT, J.K), dim3(ColCfg<MM>::NT),
+                                       colfft_lds<MM>(), s, J));
+}
+void launch_pa_select(const Job& J, hipStream_t s) {
+    if (J.W == kRangeCut) ISAR_LAUNCH(k_pa_select<kRangeCut>, dim3(J.K), dim3(kRangeCut), select_lds(J.W), s, J);
+}
+void launch_autofocus(const Job& J, hipStream_t s) {
+    ISAR_PULSE_SWITCH(J.M, ISAR_LAUNCH(k_autofocus<MM>, dim3(J.K), dim3(AfCfg<MM>::NT), autofocus_lds<MM>(), s, J));
+}
+
+}  // namespace isar

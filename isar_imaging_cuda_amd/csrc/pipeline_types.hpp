@@ -1,0 +1,80 @@
+// pipeline_types.hpp — structures shared by the host orchestrator and the kernels.
+#pragma once
+
+#ifdef ISAR_EMU
+#include "emu/hip_emu.hpp"
+#else
+#include <hip/hip_runtime.h>
+#define ISAR_LAUNCH(kernel, grid, block, smem, stream, ...) \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), (smem), (stream), __VA_ARGS__)
+#endif
+
+#include <stdint.h>
+
+namespace isar {
+
+constexpr int kRangeCut = 512;        // RANGE_NUM_CUT (common.cuh:71)
+constexpr int kRaChunkRows = 8;       // rows per workgroup in the alignment update kernel
+constexpr int kRaFinalRows = 8;       // rows per workgroup in the final alignment pass
+constexpr int kCutBandRows = 16;      // rows per workgroup when cutting the range window
+
+// Per-frame scalar decisions written by the device (one record per frame).
+struct FrameDecisions {
+    int32_t center_shift;     // circular shift applied by HRRP centring
+    int32_t center_applied;   // 1 if the centring branch ran
+    int32_t idx1_len;
+    int32_t idx2_len;
+    int32_t diff_imin;
+    int32_t cut_offset;
+    int32_t tgt_num;
+    int32_t unit2_num;
+    float center_thr;
+    float pa_threshold;
+    float gmax;
+    int32_t status;           // 0 ok; bit flags for degenerate inputs
+};
+
+// Everything a kernel needs to know about one imaging job (identical for all frames of a
+// batch). Device pointers are per batch; frame f uses the f-th slice of every buffer.
+struct Job {
+    int M, N, W, F;               // pulses, range cells, cut width, CZT length
+    int K;                        // frames in this batch
+    int data_type;                // 1 IFDS, 2 STRETCH
+    int if_hpc, if_hrrp, if_mtrc, quirk_q1, iterations;
+    int phi0_given;
+    double hpc_k;                 // HPC phase in half-turns = hpc_k * v * n^2
+    double czt_c0, czt_posa;      // alpha_n = 1 - c0 + posa * n
+    // inputs
+    const float2* echo;           // K x M x N
+    const double* velocity;       // K x M
+    const float2* phi0_in;        // K x M (optional warm start)
+    // workspaces
+    float2* pre;                  // K x M x N (IFDS pre-stage output)
+    double* lagsum;               // K x M
+    float* lags;                  // K x (M-1)
+    int32_t* imax;                // K x (M-1)
+    float* egrp;                  // K x M x N
+    float* epart;                 // K x M x N
+    float2* st;                   // K x M x N (aligned range profiles)
+    float* arp_part;              // K x ceil(M/kRaFinalRows) x N
+    float* max_part;              // K x ceil(M/kRaFinalRows)
+    FrameDecisions* dec;          // K
+    float2* x;                    // K x M x W (cut)
+    float2* rowdop;               // K x M
+    float* colmax_part;           // K x ceil(M/kCutBandRows) x W
+    float* colmax;                // K x W
+    float* colE;                  // K x W
+    float* colH;                  // K x W
+    int32_t* sel;                 // K x W
+    float2* su;                   // K x W x M (selected columns, column-major)
+    float2* phi0;                 // K x M
+    float2* phi;                  // K x M
+    float2* stm;                  // K x M x W (MTRC range-frequency rows)
+    float2* czt;                  // K x M x W (chirp-z output)
+    float2* colsum;               // K x W
+    const float2* czt_table;      // W x F (FFT of the Bluestein chirp per range bin)
+    // output
+    float* img;                   // K x M x W
+};
+
+}  // namespace isar
