@@ -107,6 +107,22 @@ ISAR_API int isar_get_trace(isar_context* ctx, int frame, isar_frame_trace* out)
  * +1 inverse); n in {64,128,192,256,384,512,1024,2048,4096,8192,16384}. Synchronous. */
 ISAR_API int isar_fft_host(int n, int dir, const float* h_in, float* h_out, int batch);
 
+/* Kernel families that can be probed with HIP events (isar_probe_enable). */
+enum isar_stage {
+    ISAR_STAGE_IFDS_PRE = 0, ISAR_STAGE_RA_INIT = 1, ISAR_STAGE_RA_CORR = 2, ISAR_STAGE_RA_UPD = 3,
+    ISAR_STAGE_RA_FINAL = 4, ISAR_STAGE_CENTER = 5, ISAR_STAGE_CUT = 6, ISAR_STAGE_PA_COLFFT = 7,
+    ISAR_STAGE_PA_SELECT = 8, ISAR_STAGE_AUTOFOCUS = 9, ISAR_STAGE_MTRC_ROWS = 10, ISAR_STAGE_MTRC_CZT = 11,
+    ISAR_STAGE_COL_WINDOW = 12, ISAR_STAGE_MTRC_IMAGE = 13, ISAR_STAGE_POST_NOMTRC = 14,
+    ISAR_STAGE_COUNT = 15
+};
+
+/* Record a HIP event pair around every launch of kernel family `stage` (on the run stream) in
+ * the following runs; stage < 0 disables. Events are created here, not during runs. */
+ISAR_API int isar_probe_enable(isar_context* ctx, int stage);
+/* Waits for the recorded events and returns the summed duration (ms) and the launch count since
+ * the last enable/read, then resets the accumulator. */
+ISAR_API int isar_probe_read(isar_context* ctx, double* total_ms, int* launches);
+
 ISAR_API const char* isar_last_error(void);
 ISAR_API int isar_device_count(void);
 

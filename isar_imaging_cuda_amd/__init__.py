@@ -1,0 +1,251 @@
+"""MI355X-native ISAR range-Doppler imaging (Python binding of libisar_imaging.so).
+
+The imaging chain is the reference's isarMainSingle path (isar_imaging_export.cpp:208-212 ->
+isar_main.cu:3-227): high-speed compensation, range alignment, HRRP centring and cut, Doppler
+tracking + fast minimum-entropy autofocus, MTRC (chirp-z keystone) and the azimuth FFT. All
+compute runs in the HIP kernels of libisar_imaging.so; this module only moves pointers. There is
+no CPU fallback: if the library is missing every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+__all__ = ["RadarParams", "Options", "Imager", "lib", "fft", "ImagingError", "library_path"]
+
+_LIB_DIR = Path(__file__).resolve().parent / "lib"
+library_path = _LIB_DIR / "libisar_imaging.so"
+
+
+class ImagingError(RuntimeError):
+    pass
+
+
+class _Params(C.Structure):
+    _fields_ = [("echo_num", C.c_int32), ("range_num", C.c_int32), ("range_num_cut", C.c_int32),
+                ("reserved", C.c_int32), ("band_width", C.c_int64), ("fc", C.c_int64), ("Tp", C.c_double),
+                ("Fs", C.c_int64)]
+
+
+class _Options(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("data_type", "option_alignment", "option_phase", "if_hrrp", "if_hpc",
+                                          "if_mtrc", "quirk_q1", "iterations", "warm_start", "reserved")]
+
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int32)
+
+
+class _Trace(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("n_pairs", "center_applied", "center_shift", "idx1_len", "idx2_len",
+                                          "diff_imin", "cut_offset", "tgt_num", "unit2_num", "status")] + \
+               [("center_thr", C.c_float), ("pa_threshold", C.c_float), ("ra_lag", _fp), ("ra_imax", _ip),
+                ("pa_colmax", _fp), ("pa_colE", _fp), ("pa_colH", _fp), ("select_bin", _ip), ("phi0", _fp),
+                ("phi", _fp)]
+
+
+# kernel families (enum isar_stage in include/isar_imaging_ext.h)
+STAGES = {"ifds_pre": 0, "ra_init": 1, "ra_corr": 2, "ra_upd": 3, "ra_final": 4, "center": 5, "cut": 6,
+          "pa_colfft": 7, "pa_select": 8, "autofocus": 9, "mtrc_rows": 10, "mtrc_czt": 11, "col_window": 12,
+          "mtrc_image": 13, "post_nomtrc": 14}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not library_path.exists():
+            raise ImagingError(f"{library_path} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(str(library_path))
+        vp = C.c_void_p
+        L.isar_default_options.argtypes = [C.POINTER(_Options)]
+        L.isar_create.argtypes = [C.POINTER(_Params), C.c_int, C.POINTER(vp)]
+        L.isar_destroy.argtypes = [vp]
+        L.isar_workspace_bytes.argtypes = [C.POINTER(_Params), C.c_int]
+        L.isar_workspace_bytes.restype = C.c_size_t
+        L.isar_run_device.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.POINTER(_Options), vp]
+        L.isar_run_host.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.POINTER(_Options)]
+        L.isar_get_trace.argtypes = [vp, C.c_int, C.POINTER(_Trace)]
+        L.isar_fft_host.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int]
+        L.isar_probe_enable.argtypes = [vp, C.c_int]
+        L.isar_probe_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.isar_last_error.restype = C.c_char_p
+        L.isar_device_count.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        msg = lib().isar_last_error()
+        raise ImagingError(f"{what} failed (rc={rc}): {msg.decode() if msg else ''}")
+
+
+@dataclass
+class RadarParams:
+    """RadarParameters of the reference (common.cuh:96-117)."""
+    echo_num: int
+    range_num: int
+    band_width: int = 1_000_000_000
+    fc: int = 10_000_000_000
+    Tp: float | None = None
+    Fs: int = 64_000_000
+    range_num_cut: int = 512
+
+    def c(self) -> _Params:
+        tp = self.Tp if self.Tp is not None else self.range_num / self.Fs
+        return _Params(self.echo_num, self.range_num, self.range_num_cut, 0, int(self.band_width), int(self.fc),
+                       float(tp), int(self.Fs))
+
+
+@dataclass
+class Options:
+    """Switches of ISAR_RD_Imaging_Main_Ku (isar_main.cu:3-4); STRETCH with HPC and MTRC by default."""
+    data_type: int = 2
+    option_alignment: int = 0
+    option_phase: int = 1
+    if_hrrp: bool = False
+    if_hpc: bool = True
+    if_mtrc: bool = True
+    quirk_q1: bool = True
+    iterations: int = 120
+    warm_start: bool = False
+
+    def c(self) -> _Options:
+        return _Options(self.data_type, self.option_alignment, self.option_phase, int(self.if_hrrp),
+                        int(self.if_hpc), int(self.if_mtrc), int(self.quirk_q1), self.iterations,
+                        int(self.warm_start), 0)
+
+
+@dataclass
+class FrameTrace:
+    n_pairs: int
+    center_applied: int
+    center_shift: int
+    idx1_len: int
+    idx2_len: int
+    diff_imin: int
+    cut_offset: int
+    tgt_num: int
+    unit2_num: int
+    status: int
+    center_thr: float
+    pa_threshold: float
+    ra_lag: np.ndarray = field(repr=False)
+    ra_imax: np.ndarray = field(repr=False)
+    pa_colmax: np.ndarray = field(repr=False)
+    pa_colE: np.ndarray = field(repr=False)
+    pa_colH: np.ndarray = field(repr=False)
+    select_bin: np.ndarray = field(repr=False)
+    phi0: np.ndarray = field(repr=False)
+    phi: np.ndarray = field(repr=False)
+
+
+class Imager:
+    """One imaging context: device workspaces for up to `max_frames` frames of one shape."""
+
+    def __init__(self, params: RadarParams, max_frames: int = 1):
+        self.params = params
+        self.max_frames = max_frames
+        self._h = C.c_void_p()
+        _check(lib().isar_create(C.byref(params.c()), max_frames, C.byref(self._h)), "isar_create")
+
+    def close(self) -> None:
+        if self._h:
+            lib().isar_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def workspace_bytes(params: RadarParams, frames: int) -> int:
+        return int(lib().isar_workspace_bytes(C.byref(params.c()), frames))
+
+    def run_host(self, echo: np.ndarray, velocity: np.ndarray | None, options: Options | None = None,
+                 phi0: np.ndarray | None = None) -> np.ndarray:
+        """echo: complex64 (K, M, N) or (M, N). Returns float32 images (K, M, 512)."""
+        options = options or Options()
+        p = self.params
+        e = np.ascontiguousarray(echo, dtype=np.complex64)
+        squeeze = e.ndim == 2
+        if squeeze:
+            e = e[None]
+        K = e.shape[0]
+        if e.shape[1:] != (p.echo_num, p.range_num):
+            raise ValueError(f"echo shape {e.shape} does not match ({p.echo_num}, {p.range_num})")
+        v = None if velocity is None else np.ascontiguousarray(
+            np.broadcast_to(velocity, (K, p.echo_num)), dtype=np.float64)
+        ph = None if phi0 is None else np.ascontiguousarray(np.broadcast_to(phi0, (K, p.echo_num)),
+                                                            dtype=np.complex64)
+        img = np.zeros((K, p.echo_num, p.range_num_cut), np.float32)
+        ptr = lambda a: None if a is None else a.ctypes.data_as(C.c_void_p)
+        _check(lib().isar_run_host(self._h, K, ptr(e), ptr(v), ptr(ph), ptr(img), C.byref(options.c())),
+               "isar_run_host")
+        return img[0] if squeeze else img
+
+    def run_device(self, echo_ptr: int, velocity_ptr: int | None, img_ptr: int, frames: int,
+                   options: Options | None = None, phi0_ptr: int | None = None, stream: int | None = None) -> None:
+        """Device-resident batch (raw device pointers, e.g. torch tensor data_ptr()); asynchronous."""
+        options = options or Options()
+        _check(lib().isar_run_device(self._h, frames, C.c_void_p(echo_ptr), C.c_void_p(velocity_ptr or 0),
+                                     C.c_void_p(phi0_ptr or 0), C.c_void_p(img_ptr), C.byref(options.c()),
+                                     C.c_void_p(stream or 0)), "isar_run_device")
+
+    def probe_enable(self, stage: int) -> None:
+        """Time every launch of kernel family `stage` (STAGES) with HIP events on the run stream."""
+        _check(lib().isar_probe_enable(self._h, int(stage)), "isar_probe_enable")
+
+    def probe_read(self) -> tuple[float, int]:
+        """(summed milliseconds, launches) since the last enable/read."""
+        ms, n = C.c_double(), C.c_int()
+        _check(lib().isar_probe_read(self._h, C.byref(ms), C.byref(n)), "isar_probe_read")
+        return ms.value, n.value
+
+    def trace(self, frame: int = 0) -> FrameTrace:
+        M, W = self.params.echo_num, self.params.range_num_cut
+        lag = np.zeros(M - 1, np.float32)
+        imax = np.zeros(M - 1, np.int32)
+        cm, ce, ch = (np.zeros(W, np.float32) for _ in range(3))
+        sel = np.full(W, -1, np.int32)
+        p0 = np.zeros(2 * M, np.float32)
+        p1 = np.zeros(2 * M, np.float32)
+        t = _Trace()
+        t.ra_lag = lag.ctypes.data_as(_fp)
+        t.ra_imax = imax.ctypes.data_as(_ip)
+        t.pa_colmax = cm.ctypes.data_as(_fp)
+        t.pa_colE = ce.ctypes.data_as(_fp)
+        t.pa_colH = ch.ctypes.data_as(_fp)
+        t.select_bin = sel.ctypes.data_as(_ip)
+        t.phi0 = p0.ctypes.data_as(_fp)
+        t.phi = p1.ctypes.data_as(_fp)
+        _check(lib().isar_get_trace(self._h, frame, C.byref(t)), "isar_get_trace")
+        sc = {n: getattr(t, n) for n, _ in _Trace._fields_[:12]}
+        return FrameTrace(**sc, ra_lag=lag, ra_imax=imax, pa_colmax=cm, pa_colE=ce, pa_colH=ch,
+                          select_bin=sel[: t.unit2_num].copy(), phi0=p0[0::2] + 1j * p0[1::2],
+                          phi=p1[0::2] + 1j * p1[1::2])
+
+
+def fft(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    """Unnormalised device FFT of each row of a 2-D complex array (tests and diagnostics)."""
+    a = np.ascontiguousarray(x, dtype=np.complex64)
+    if a.ndim == 1:
+        a = a[None]
+    out = np.zeros_like(a)
+    _check(lib().isar_fft_host(a.shape[1], 1 if inverse else -1, a.ctypes.data_as(C.c_void_p),
+                               out.ctypes.data_as(C.c_void_p), a.shape[0]), "isar_fft_host")
+    return out

@@ -96,7 +96,7 @@ def build_imaging(force: bool = True, jobs: int = 8) -> Path | None:
                 _run([HIPCC, *HIP_FLAGS, *incs, "-c", str(src), "-o", str(obj)])
             else:
                 _run([HIPCC, "-O3", "-std=c++20", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__", *incs,
-                      "-x", "c++", "-c", str(src), "-o", str(obj)])
+                      "-I", "/opt/rocm/include", "-x", "c++", "-c", str(src), "-o", str(obj)])
         return obj
 
     srcs = hip + _host_sources() + [CSRC / "host" / "synth.cpp"]

@@ -1,0 +1,88 @@
+// isar_calling_app.cpp — the reference's sample caller (calling_dll/calling_dll.cpp:17-89) against
+// the drop-in library: gpuDevInit -> dataParsing -> imagingMemInit -> { dataExtracting ->
+// isarMainSingle } x n -> imagingMemDest. Built against include/isar_imaging_export.h only.
+//
+// usage: isar_calling_app <dir> <window_len> <imaging_stride> <n_images> <out_prefix>
+//                         [polar_type=1] [if_mtrc=1] [if_hpc=1]
+// writes <out_prefix>_<i>.bin (raw float32 image, window_len x 512) and <out_prefix>_0.txt.
+// n_images = 0 only parses the directory (no GPU call) and prints the decoded headers.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <string>
+
+#include "isar_imaging_export.h"
+#include "isar_imaging_ext.h"
+
+int main(int argc, char** argv) {
+    if (argc < 6) {
+        std::fprintf(stderr, "usage: %s dir window_len imaging_stride n_images out_prefix [polar] [mtrc] [hpc]\n", argv[0]);
+        return 2;
+    }
+    const std::string dir_path = argv[1];
+    int window_len = std::atoi(argv[2]);
+    int imaging_stride = std::atoi(argv[3]);
+    const int n_images = std::atoi(argv[4]);
+    const std::string prefix = argv[5];
+    const int polar_type = argc > 6 ? std::atoi(argv[6]) : 1;
+    const bool if_mtrc = argc > 7 ? std::atoi(argv[7]) != 0 : true;
+    const bool if_hpc = argc > 8 ? std::atoi(argv[8]) != 0 : true;
+    const int data_type = 2;          // STRETCH
+    const int sampling_stride = 1;
+    int window_head = 0;
+    const int option_alignment = 0, option_phase = 1;
+    const bool if_hrrp = false;
+
+    vec1D_DBL dataN;
+    vec1D_FLT turnAngle;
+    int frame_len = 0, frame_num = 0;
+    vec1D_INT dataWFileSn;
+    vec1D_DBL dataNOut;
+    vec1D_FLT turnAngleOut;
+    vec1D_COM_FLT dataW;
+    vec1D_FLT img;
+
+    if (n_images > 0 && gpuDevInit() != EXIT_SUCCESS) {
+        std::fprintf(stderr, "gpuDevInit: %s\n", isar_last_error());
+        return 3;
+    }
+    if (dataParsing(&dataN, &turnAngle, &frame_len, &frame_num, dir_path, polar_type, data_type) != EXIT_SUCCESS) {
+        std::fprintf(stderr, "dataParsing: %s\n", isar_last_error());
+        return 4;
+    }
+    if (n_images == 0) {
+        const size_t total = dataN.size() / 4;
+        std::printf("frame_len %d frame_num %d frames %zu\n", frame_len, frame_num, total);
+        for (size_t i = 0; i < total; ++i)
+            std::printf("%zu %.6f %.6f %.9f %.9f %.7f\n", i, dataN[i], dataN[i + total], dataN[i + 2 * total],
+                        dataN[i + 3 * total], turnAngle[i]);
+        return 0;
+    }
+    imagingMemInit(&img, &dataWFileSn, &dataNOut, &turnAngleOut, &dataW, window_len, frame_len, data_type, if_hpc, if_hrrp);
+    if (img.size() != static_cast<size_t>(window_len) * 512) {
+        std::fprintf(stderr, "imagingMemInit: %s\n", isar_last_error());
+        return 5;
+    }
+    for (int i = 0; i < n_images; ++i) {
+        auto t1 = std::chrono::high_resolution_clock::now();
+        if (dataExtracting(&dataWFileSn, &dataNOut, &turnAngleOut, &dataW, dataN, turnAngle, frame_len, frame_num,
+                           sampling_stride, window_head, window_len, imaging_stride, data_type) != EXIT_SUCCESS) {
+            std::fprintf(stderr, "dataExtracting: %s\n", isar_last_error());
+            return 6;
+        }
+        auto t2 = std::chrono::high_resolution_clock::now();
+        isarMainSingle(img.data(), data_type, option_alignment, option_phase, if_hrrp, if_hpc, if_mtrc);
+        auto t3 = std::chrono::high_resolution_clock::now();
+        std::printf("[img %2d] %3dms / %3dms\n", i + 1,
+                    static_cast<int>(std::chrono::duration_cast<std::chrono::milliseconds>(t2 - t1).count()),
+                    static_cast<int>(std::chrono::duration_cast<std::chrono::milliseconds>(t3 - t2).count()));
+        std::ofstream out(prefix + "_" + std::to_string(i) + ".bin", std::ios::binary);
+        out.write(reinterpret_cast<const char*>(img.data()), static_cast<std::streamsize>(img.size() * sizeof(float)));
+        if (i == 0) writeFileFLT(prefix + "_0.txt", img.data(), img.size());
+        window_head += imaging_stride;
+    }
+    imagingMemDest(data_type, if_hpc, if_hrrp);
+    std::printf("turn angle of the last window: %.6f .. %.6f\n", turnAngleOut.front(), turnAngleOut.back());
+    return 0;
+}

@@ -303,7 +303,7 @@ __device__ __forceinline__ void stockham_compute(float2 (&x)[E], int tid) {
             for (int r = 2; r < R; ++r) {
                 if ((r & 1) == 0 && r <= 8) w[r] = expjpi(DIR * base * static_cast<float>(r));
                 else if (r < 8) w[r] = cmul(w[r - 1], w[1]);
-                else w[r] = cmul(w[8], w[r - 8]);
+                else w[r] = cmul(w[8 % R], w[(r - 8) % R]);   // reached only for R = 16
             }
 #pragma unroll
             for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);

@@ -36,6 +36,12 @@ const char* last_error_cstr() {
     return copy.c_str();
 }
 
+std::string g_config_error;
+void report_config_error(const char* file, size_t lds_bytes) {
+    if (g_config_error.empty())
+        g_config_error = std::string(file) + ": dynamic LDS of " + std::to_string(lds_bytes) + " bytes rejected";
+}
+
 hipError_t configure_kernels() {
     static std::once_flag once;
     static hipError_t result = hipSuccess;
@@ -90,7 +96,44 @@ size_t workspace_bytes(const isar_radar_params& p, int K) {
 }
 
 Context::~Context() {
+    for (hipEvent_t e : probe_events) (void)hipEventDestroy(e);
     for (void* a : allocations) (void)hipFree(a);
+}
+
+int Context::probe_enable(int stage) {
+    probe_collect();
+    probe_stage = stage;
+    probe_ms = 0.0;
+    probe_launches = 0;
+    if (stage >= 0 && probe_events.empty()) {
+        probe_events.resize(2 * 512);
+        for (auto& e : probe_events)
+            if (hipEventCreate(&e) != hipSuccess) return fail(ISAR_ERR_DEVICE, "hipEventCreate failed");
+    }
+    return ISAR_OK;
+}
+
+int Context::probe_collect() {
+    if (probe_used == 0) return ISAR_OK;
+    for (int i = 0; i < probe_used; ++i) {
+        if (hipEventSynchronize(probe_events[2 * i + 1]) != hipSuccess) return fail(ISAR_ERR_DEVICE, "probe sync");
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, probe_events[2 * i], probe_events[2 * i + 1]) != hipSuccess)
+            return fail(ISAR_ERR_DEVICE, "probe elapsed time");
+        probe_ms += ms;
+    }
+    probe_launches += probe_used;
+    probe_used = 0;
+    return ISAR_OK;
+}
+
+int Context::probe_read(double* ms, int* launches) {
+    int rc = probe_collect();
+    if (ms) *ms = probe_ms;
+    if (launches) *launches = probe_launches;
+    probe_ms = 0.0;
+    probe_launches = 0;
+    return rc;
 }
 
 int Context::init(const isar_radar_params& params, int frames) {
@@ -98,7 +141,8 @@ int Context::init(const isar_radar_params& params, int frames) {
     if (rc) return rc;
     if (frames < 1) return fail(ISAR_ERR_INVALID, "max_frames must be >= 1");
     hipError_t ce = configure_kernels();
-    if (ce != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("kernel configuration: ") + hipGetErrorString(ce));
+    if (ce != hipSuccess)
+        return fail(ISAR_ERR_DEVICE, std::string("kernel configuration: ") + hipGetErrorString(ce) + " " + g_config_error);
     p = params;
     max_frames = frames;
     const size_t K = frames, M = p.echo_num, N = p.range_num, W = p.range_num_cut;
@@ -182,29 +226,31 @@ int Context::run(int frames, const void* echo, const double* velocity, const voi
     const int M = J.M;
 
     (void)hipGetLastError();
-    if (J.data_type == 1) launch_ifds_pre(J, s);
-    launch_ra_init(J, s);
+    // a family launches at most log2(M) + 1 times per run; drain the ring only when it could overflow
+    if (probe_stage >= 0 && probe_used + 16 > static_cast<int>(probe_events.size()) / 2) probe_collect();
+    if (J.data_type == 1) probed(ISAR_STAGE_IFDS_PRE, s, [&] { launch_ifds_pre(J, s); });
+    probed(ISAR_STAGE_RA_INIT, s, [&] { launch_ra_init(J, s); });
     int pair_base = 0;
     for (int st = 1; st < M; st *= 2) {
         const int ngroups = (M + st - 1) / st, npairs = ngroups / 2, unpaired = ngroups & 1;
-        launch_ra_corr(J, st, pair_base, npairs + unpaired, s);
+        probed(ISAR_STAGE_RA_CORR, s, [&] { launch_ra_corr(J, st, pair_base, npairs + unpaired, s); });
         const int R = std::min(st, kRaChunkRows), cpp = (st + R - 1) / R;
-        if (npairs > 0) launch_ra_upd(J, st, pair_base, npairs * cpp, s);
+        if (npairs > 0) probed(ISAR_STAGE_RA_UPD, s, [&] { launch_ra_upd(J, st, pair_base, npairs * cpp, s); });
         pair_base += npairs;
     }
-    launch_ra_final(J, s);
-    launch_center(J, s);
-    launch_cut(J, s);
-    launch_pa_colfft(J, s);
-    launch_pa_select(J, s);
-    launch_autofocus(J, s);
+    probed(ISAR_STAGE_RA_FINAL, s, [&] { launch_ra_final(J, s); });
+    probed(ISAR_STAGE_CENTER, s, [&] { launch_center(J, s); });
+    probed(ISAR_STAGE_CUT, s, [&] { launch_cut(J, s); });
+    probed(ISAR_STAGE_PA_COLFFT, s, [&] { launch_pa_colfft(J, s); });
+    probed(ISAR_STAGE_PA_SELECT, s, [&] { launch_pa_select(J, s); });
+    probed(ISAR_STAGE_AUTOFOCUS, s, [&] { launch_autofocus(J, s); });
     if (J.if_mtrc) {
-        launch_mtrc_rows(J, s);
-        launch_mtrc_czt(J, s);
-        if (!J.quirk_q1) launch_col_window(J, s);
-        launch_mtrc_image(J, s);
+        probed(ISAR_STAGE_MTRC_ROWS, s, [&] { launch_mtrc_rows(J, s); });
+        probed(ISAR_STAGE_MTRC_CZT, s, [&] { launch_mtrc_czt(J, s); });
+        if (!J.quirk_q1) probed(ISAR_STAGE_COL_WINDOW, s, [&] { launch_col_window(J, s); });
+        probed(ISAR_STAGE_MTRC_IMAGE, s, [&] { launch_mtrc_image(J, s); });
     } else {
-        launch_post_nomtrc(J, s);
+        probed(ISAR_STAGE_POST_NOMTRC, s, [&] { launch_post_nomtrc(J, s); });
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -343,7 +389,7 @@ ISAR_API int isar_get_trace(isar_context* ctx, int frame, isar_frame_trace* out)
 ISAR_API int isar_fft_host(int n, int dir, const float* h_in, float* h_out, int batch) {
     if (!h_in || !h_out || batch < 1 || (dir != -1 && dir != 1)) return fail(ISAR_ERR_INVALID, "bad fft arguments");
     hipError_t ce = isar::configure_kernels();
-    if (ce != hipSuccess) return fail(ISAR_ERR_DEVICE, "kernel configuration failed");
+    if (ce != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("kernel configuration failed: ") + isar::g_config_error);
     const size_t bytes = static_cast<size_t>(n) * batch * sizeof(float2);
     void *a = nullptr, *b = nullptr;
     if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
@@ -362,6 +408,17 @@ ISAR_API int isar_fft_host(int n, int dir, const float* h_in, float* h_out, int 
     if (!ok) return fail(ISAR_ERR_INVALID, "unsupported fft length " + std::to_string(n));
     if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("fft: ") + hipGetErrorString(e));
     return ISAR_OK;
+}
+
+ISAR_API int isar_probe_enable(isar_context* ctx, int stage) {
+    if (!ctx) return fail(ISAR_ERR_INVALID, "null context");
+    if (stage >= ISAR_STAGE_COUNT) return fail(ISAR_ERR_INVALID, "unknown stage");
+    return ctx->probe_enable(stage);
+}
+
+ISAR_API int isar_probe_read(isar_context* ctx, double* total_ms, int* launches) {
+    if (!ctx) return fail(ISAR_ERR_INVALID, "null context");
+    return ctx->probe_read(total_ms, launches);
 }
 
 ISAR_API const char* isar_last_error(void) { return isar::last_error_cstr(); }

@@ -20,6 +20,26 @@ struct Context {
     std::vector<void*> allocations;   // everything allocated at creation
     float2* czt_table = nullptr;
     int last_frames = 0;
+    // kernel probe: event pairs recorded around each launch of one kernel family
+    int probe_stage = -1;
+    std::vector<hipEvent_t> probe_events;   // 2 per slot
+    int probe_used = 0;
+    double probe_ms = 0.0;
+    int probe_launches = 0;
+    int probe_enable(int stage);
+    int probe_read(double* ms, int* launches);
+    int probe_collect();
+    template <typename F>
+    void probed(int stage, hipStream_t s, F&& launch) {
+        if (stage == probe_stage && probe_used * 2 + 1 < static_cast<int>(probe_events.size())) {
+            (void)hipEventRecord(probe_events[2 * probe_used], s);
+            launch();
+            (void)hipEventRecord(probe_events[2 * probe_used + 1], s);
+            ++probe_used;
+        } else {
+            launch();
+        }
+    }
     // device staging for isar_run_host: echo, velocity, phi0, image (allocated on first use)
     void* stage_dev[4] = {nullptr, nullptr, nullptr, nullptr};
 

@@ -180,8 +180,11 @@ __global__ __launch_bounds__(W) void k_cut(Job J) {
 
 hipError_t configure_center_kernels() {
 #ifndef ISAR_EMU
-    return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_center), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               static_cast<int>((2 * 16384 + 64) * sizeof(float)));
+    const size_t bytes = (2 * 16384 + 64) * sizeof(float);
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_center), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       static_cast<int>(bytes));
+    if (e != hipSuccess) report_config_error(__FILE__, bytes);
+    return e;
 #else
     return hipSuccess;
 #endif

@@ -33,7 +33,10 @@ hipError_t configure_debug_kernels() {
 #ifndef ISAR_EMU
     auto set = [&](const void* fn, size_t bytes) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
-        if (e != hipSuccess) err = e;
+        if (e != hipSuccess) {
+            err = e;
+            report_config_error(__FILE__, bytes);
+        }
     };
     set(reinterpret_cast<const void*>(&k_debug_fft<16384, 16, -1>), lds_padded_size(16384) * sizeof(float2));
     set(reinterpret_cast<const void*>(&k_debug_fft<16384, 16, +1>), lds_padded_size(16384) * sizeof(float2));

@@ -324,11 +324,14 @@ hipError_t configure_mtrc_kernels() {
 #ifndef ISAR_EMU
     auto set = [&](const void* fn, size_t bytes) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
-        if (e != hipSuccess) err = e;
+        if (e != hipSuccess) {
+            err = e;
+            report_config_error(__FILE__, bytes);
+        }
     };
 #define ISAR_SET_F(FF)                                                                          \
     set(reinterpret_cast<const void*>(&k_czt_table<FF>), lds_padded_size(FF) * sizeof(float2)); \
-    set(reinterpret_cast<const void*>(&k_mtrc_czt<FF>), czt_lds<FF>(1024));
+    set(reinterpret_cast<const void*>(&k_mtrc_czt<FF>), czt_lds<FF>(FF / 2));
     ISAR_SET_F(128) ISAR_SET_F(256) ISAR_SET_F(512) ISAR_SET_F(1024) ISAR_SET_F(2048)
 #undef ISAR_SET_F
 #define ISAR_SET_M(MM)                                                         \

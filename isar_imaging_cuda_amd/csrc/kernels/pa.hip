@@ -110,8 +110,8 @@ __global__ __launch_bounds__(256) void k_pa_colfft(Job J) {
 // sorted by index. One workgroup of W threads per frame.
 template <int W>
 __global__ __launch_bounds__(W) void k_pa_select(Job J) {
-    HIP_DYNAMIC_SHARED(float, sm);
-    float* key = sm;                                  // W
+    HIP_DYNAMIC_SHARED(float, sm_sel);
+    float* key = sm_sel;                              // W
     int* tag = reinterpret_cast<int*>(key + W);       // W
     int* order1 = tag + W;                            // W
     float* red = reinterpret_cast<float*>(order1 + W);   // 32
@@ -301,7 +301,10 @@ hipError_t configure_pa_kernels() {
 #ifndef ISAR_EMU
     auto set = [&](const void* fn, size_t bytes) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
-        if (e != hipSuccess) err = e;
+        if (e != hipSuccess) {
+            err = e;
+            report_config_error(__FILE__, bytes);
+        }
     };
 #define ISAR_SET_PA(MM)                                                          \
     set(reinterpret_cast<const void*>(&k_pa_colfft<MM>), colfft_lds<MM>());      \

@@ -265,7 +265,10 @@ hipError_t configure_ra_kernels() {
 #ifndef ISAR_EMU
     auto set = [&](const void* fn, size_t bytes) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes));
-        if (e != hipSuccess) err = e;
+        if (e != hipSuccess) {
+            err = e;
+            report_config_error(__FILE__, bytes);
+        }
     };
 #define ISAR_SET_RA(NN)                                                        \
     set(reinterpret_cast<const void*>(&k_ifds_pre<NN>), ra_lds<NN>());         \

@@ -11,6 +11,8 @@ bool pulse_count_supported(int M);
 
 // Sets the dynamic-LDS limits of every kernel instantiation (call once per process/device).
 hipError_t configure_kernels();
+// Records which kernel family could not be configured (for isar_last_error).
+void report_config_error(const char* file, size_t lds_bytes);
 
 // range alignment (range_alignment.cu:4-78)
 void launch_ifds_pre(const Job& J, hipStream_t s);

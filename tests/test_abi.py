@@ -1,0 +1,90 @@
+"""CPU checks of the C-ABI library: every declared symbol is exported, the library loads, the
+argument validation returns error codes, and the .wbd ingest decodes headers (no GPU needed)."""
+from __future__ import annotations
+
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import isar_imaging_cuda_amd as isar
+from isar_imaging_cuda_amd import synth
+
+ROOT = Path(__file__).resolve().parents[1]
+APP = ROOT / "isar_imaging_cuda_amd" / "lib" / "isar_calling_app"
+
+
+def _declared(header: str) -> list[str]:
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"(?:ISAR_API|DLL_EXPORT_API)\s+[\w\s\*]+?\b(\w+)\s*\(", text)
+    names += re.findall(r"^\s*(?:int|void|size_t)\s+\**\s*(isar_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+@pytest.mark.parametrize("header", ["isar_imaging_export.h", "isar_imaging_ext.h", "isar_synth.h"])
+def test_library_exports_every_declared_entry_point(header):
+    names = _declared(header)
+    assert names, header
+    lib = isar.lib()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, f"{header}: {missing}"
+
+
+def test_drop_in_names_are_unmangled():
+    out = subprocess.run(["nm", "-D", "--defined-only", str(isar.library_path)], capture_output=True, text=True).stdout
+    for name in ("gpuDevInit", "dataParsing", "dataExtracting", "imagingMemInit", "isarMainSingle", "writeFileFLT",
+                 "imagingMemDest"):
+        assert re.search(rf"\bT {name}$", out, flags=re.M), name
+
+
+def test_invalid_sizes_return_error_codes():
+    # imagingMemInit validation (isar_imaging_export.cpp:144-155) without touching a device
+    for M, N in [(2048, 4096), (256, 256), (1000, 4096), (256, 3000)]:
+        with pytest.raises(isar.ImagingError) as e:
+            isar.Imager(isar.RadarParams(M, N))
+        assert "rc=-1" in str(e.value)
+    msg = isar.lib().isar_last_error()
+    assert msg and len(msg) > 0
+
+
+def test_workspace_bytes_scales_with_frames():
+    p = isar.RadarParams(256, 4096)
+    one, eight = isar.Imager.workspace_bytes(p, 1), isar.Imager.workspace_bytes(p, 8)
+    assert one > 256 * 4096 * 8 and eight > 7 * one
+
+
+def test_wbd_ingest_decodes_headers(tmp_path):
+    from wbd import make_dwell, read_wbd
+    path, echo, vel, cfg = make_dwell(tmp_path, pulses=64, range_num=1024, azimuth_rate_deg=0.05)
+    ref = read_wbd(path)
+    assert np.array_equal(ref["echo"], echo)                      # samples were already int16-valued
+    r = subprocess.run([str(APP), str(tmp_path), "64", "16", "0", str(tmp_path / "x"), "1"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().splitlines()
+    fl, fn = int(lines[0].split()[1]), int(lines[0].split()[3])
+    assert fl == 256 + 4 * 1024 and fn == 64
+    vals = np.array([[float(v) for v in ln.split()] for ln in lines[1:]])
+    assert np.allclose(vals[:, 1], ref["range"], atol=1e-6)
+    assert np.allclose(vals[:, 2], ref["vel"], atol=1e-6)
+    assert np.allclose(vals[:, 3], ref["az"], atol=1e-8)
+    assert np.allclose(vals[:, 4], ref["pitch"], atol=1e-8)
+    assert np.allclose(ref["vel"], np.round(vel * 10) / 10, atol=1e-9)   # 0.1 m/s quantisation
+    # turn angle (common.cu:902-1020) for a constant pitch p and azimuth rate r: great-circle
+    # angles r cos(p) per pulse accumulated at 20-pulse knots 0, 20, 40, linear between knots and
+    # held constant after the last knot (no extrapolation)
+    i = np.arange(64)
+    expect = 0.05 * np.cos(np.deg2rad(5.0)) * np.minimum(i, 40)
+    assert np.abs(vals[:, 5] - expect).max() < 2e-3 * expect.max()
+
+
+def test_wbd_other_polarisation_is_not_picked_up(tmp_path):
+    from wbd import make_dwell
+    make_dwell(tmp_path, pulses=32, range_num=1024, polar="1100")
+    r = subprocess.run([str(APP), str(tmp_path), "32", "8", "0", str(tmp_path / "x"), "1"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 4 and "no valid file" in r.stderr

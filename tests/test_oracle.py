@@ -1,0 +1,337 @@
+"""Pins the FP64 oracle (the checker of the HIP path) against closed forms, golden vectors made
+by independent libraries (tests/golden/make_golden.py) and known-answer constructions."""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from isar_imaging_cuda_amd import synth
+from oracle import oracle as O
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+# ---------------------------------------------------------------------------------------------
+# FFT (common to every stage): brute-force DFT, numpy fixtures, Parseval, known answers
+# ---------------------------------------------------------------------------------------------
+def _dft(x, sign):
+    n = x.size
+    k = np.arange(n)
+    return np.exp(sign * 2j * np.pi * np.outer(k, k) / n) @ x
+
+
+@pytest.mark.parametrize("n", [8, 12, 48, 64, 192, 256, 384])
+def test_fft_matches_bruteforce_dft(n):
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert np.allclose(O.fft(x), _dft(x, -1), rtol=0, atol=1e-10 * n)
+    assert np.allclose(O.fft(x, inverse=True), _dft(x, +1), rtol=0, atol=1e-10 * n)
+
+
+def test_fft_matches_numpy_fixtures():
+    g = np.load(GOLD / "fft_vectors.npz")
+    for key in g.files:
+        if not key.startswith("x_"):
+            continue
+        n = int(key[2:])
+        x = g[key]
+        assert np.allclose(O.fft(x), g[f"fwd_{n}"], atol=1e-9 * n), n
+        assert np.allclose(O.fft(x, inverse=True), g[f"inv_{n}"], atol=1e-9 * n), n
+
+
+def test_fft_known_answers():
+    n = 256
+    x = np.zeros(n, complex)
+    x[0] = 1
+    assert np.allclose(O.fft(x), np.ones(n))                        # impulse -> flat
+    k0 = 17
+    tone = np.exp(2j * np.pi * k0 * np.arange(n) / n)
+    X = O.fft(tone)
+    assert abs(X[k0] - n) < 1e-9 and np.abs(np.delete(X, k0)).max() < 1e-9   # tone -> one bin
+    y = np.random.default_rng(1).standard_normal(n) + 0j
+    assert np.isclose(np.sum(np.abs(O.fft(y)) ** 2), n * np.sum(np.abs(y) ** 2))   # Parseval
+
+
+def test_next_pow2_reference_semantics():
+    # common.cu:521-529 returns 2^(floor(log2 n) + 1): exact powers of two double
+    assert [O.next_pow2(v) for v in (0, 1, 2, 3, 255, 256, 257, 511, 512)] == [2, 2, 4, 4, 256, 512, 512, 512, 1024]
+
+
+# ---------------------------------------------------------------------------------------------
+# chirp-z transform (mtrc.cu:125-192)
+# ---------------------------------------------------------------------------------------------
+def test_czt_matches_scipy_fixtures():
+    g = np.load(GOLD / "czt_vectors.npz")
+    i = 0
+    while f"x_{i}" in g.files:
+        x, w, a, out, alpha = g[f"x_{i}"], complex(g[f"w_{i}"]), complex(g[f"a_{i}"]), g[f"out_{i}"], float(g[f"alpha_{i}"])
+        got = O.czt(x, x.size, w, a)
+        assert np.allclose(got, out, rtol=1e-9, atol=1e-9 * np.abs(out).max()), i
+        if not np.isnan(alpha):
+            got2 = O.czt_alpha(x, alpha)
+            assert np.allclose(got2, out, rtol=1e-9, atol=1e-9 * np.abs(out).max()), i
+        i += 1
+    assert i >= 6
+
+
+def test_czt_alpha_one_is_shifted_dft():
+    # alpha = 1: w = exp(-j 2 pi / M), a = exp(-j pi) -> X[k] = DFT[(k - M/2) mod M]
+    m = 128
+    x = np.random.default_rng(3).standard_normal(m) + 1j * np.random.default_rng(4).standard_normal(m)
+    assert np.allclose(O.czt_alpha(x, 1.0), np.fft.fftshift(np.fft.fft(x)), atol=1e-9)
+
+
+# ---------------------------------------------------------------------------------------------
+# high-speed compensation (high_speed_compensation.cu:4-54)
+# ---------------------------------------------------------------------------------------------
+def test_hpc_closed_form():
+    M, N = 8, 1024
+    p = O.params(M, N, 512)
+    rng = np.random.default_rng(5)
+    s = rng.standard_normal((M, N)) + 1j * rng.standard_normal((M, N))
+    v = rng.uniform(-3000, 3000, M)
+    out = O.hpc(s, v, p)
+    K = p.band_width / p.Tp
+    tk = np.arange(N) / p.Fs
+    expect = s * np.exp(1j * (-4 * np.pi * K / 3e8) * v[:, None] * tk[None, :] ** 2)
+    assert np.allclose(out, expect, atol=1e-10)
+    # the sign convention matters: the conjugate correction must not match
+    wrong = s * np.exp(-1j * (-4 * np.pi * K / 3e8) * v[:, None] * tk[None, :] ** 2)
+    assert not np.allclose(out, wrong, atol=1e-3)
+
+
+def test_hpc_removes_synthetic_intrapulse_term():
+    cfg = synth.preset(2, echo_num=64, range_num=1024, n_scatterers=1, box_x=0.0, box_y=0.0, box_z=0.0,
+                       walk_cells=0.0, phase_error_model=0, snr_db=300.0, peak_adc=0.0, quantize=0, elevation=0.0,
+                       omega=0.0)
+    echo, vel = synth.frame(cfg)
+    h = synth.radar_header(cfg)
+    p = O.params(64, 1024, 512, h["band_width"], h["fc"], h["Tp"], h["Fs"])
+    out = O.hpc(echo, vel, p)
+    # the generator puts (4 pi K / c) v_m t^2 into every pulse; HPC removes it exactly, leaving a
+    # pure tone in fast time (quadratic phase coefficient ~ 0)
+    n = np.arange(1024, dtype=np.float64)
+    expect = (4 * np.pi * (h["band_width"] / h["Tp"]) / 3e8) * vel[10] / h["Fs"] ** 2
+    c_raw = np.polyfit(n, np.unwrap(np.angle(echo[10].astype(np.complex128))), 2)[0]
+    c_hpc = np.polyfit(n, np.unwrap(np.angle(out[10])), 2)[0]
+    assert abs(c_raw - expect) < 1e-3 * expect
+    assert abs(c_hpc) < 1e-3 * expect
+
+
+# ---------------------------------------------------------------------------------------------
+# range alignment (range_alignment.cu:4-201)
+# ---------------------------------------------------------------------------------------------
+def _profiles_with_shifts(M, N, shifts, seed=0, speckle=True, width=(1, 4)):
+    """Frequency-domain rows whose range profiles are one template shifted by `shifts` cells."""
+    rng = np.random.default_rng(seed)
+    n = np.arange(N)
+    prof = np.zeros(N)
+    for c, a, wdt in zip(rng.uniform(N / 2 - 60, N / 2 + 60, 12), rng.uniform(0.5, 2, 12), rng.uniform(*width, 12)):
+        prof += a * np.exp(-0.5 * ((n - c) / wdt) ** 2)
+    phases = np.exp(1j * rng.uniform(0, 2 * np.pi, N)) if speckle else 1.0   # random phases per cell
+    base = np.fft.fft(prof * phases)
+    k = np.arange(N)
+    S = np.stack([base * np.exp(2j * np.pi * k * s / N) for s in shifts])
+    # undo the centring the oracle applies before its IFFT: hamming * (-1)^n
+    ham = 0.54 - 0.46 * np.cos(2 * np.pi * np.arange(N) / (N - 1))
+    # range profiles in the oracle are IFFT(S * w (-1)^n): build S so IFFT(S) is the profile
+    return np.fft.fft(np.fft.ifft(S, axis=1), axis=1) / (ham * (-1.0) ** np.arange(N)), prof
+
+
+def test_range_alignment_recovers_integer_shifts():
+    M, N = 16, 1024
+    rng = np.random.default_rng(7)
+    shifts = rng.integers(-20, 21, M)
+    S, _ = _profiles_with_shifts(M, N, shifts)
+    p = O.params(M, N, 512)
+    tr = O.Trace(M, N, 512)
+    out = O.range_alignment(S, p, tr)
+    assert tr.n_pairs == M - 1
+    prof = np.abs(out)
+    # every aligned profile coincides with row 0 (the merge tree never moves row 0)
+    for m in range(1, M):
+        c = np.fft.ifft(np.fft.fft(prof[0]) * np.conj(np.fft.fft(prof[m]))).real
+        assert np.argmax(c) == 0, f"row {m} left misaligned by {np.argmax(c)}"
+    # the level-0 lags are the pairwise shift differences
+    for pp in range(M // 2):
+        assert abs(tr.ra_lag[pp] - (shifts[2 * pp + 1] - shifts[2 * pp])) < 0.35
+
+
+@pytest.mark.parametrize("shift", [3.4, -7.25, 0.5])
+def test_range_alignment_fractional_shift(shift):
+    # dechirped rows are sums of beat tones exp(j 2 pi beta k / N); a target moved by s cells
+    # has beta + s. The lag of the pair is the fractional shift between the two rows.
+    M, N = 2, 1024
+    rng = np.random.default_rng(2)
+    k = np.arange(N)
+    betas, amps = rng.uniform(-60, 60, 8), rng.uniform(0.5, 2, 8) * np.exp(1j * rng.uniform(0, 6.28, 8))
+    S = np.stack([sum(a * np.exp(2j * np.pi * (b + s) * k / N) for a, b in zip(amps, betas)) for s in (0.0, shift)])
+    tr = O.Trace(M, N, 512)
+    O.range_alignment(S, O.params(M, N, 512), tr)
+    assert abs(tr.ra_lag[0] - shift) < 0.1
+
+
+def test_parabolic_vertex_closed_form():
+    # binomialFixDevice: the vertex of the parabola through (i-1, f1), (i, f2), (i+1, f3)
+    for d in (-0.4, -0.1, 0.0, 0.27, 0.49):
+        f = lambda t: 5.0 - 2.0 * (t - d) ** 2
+        f1, f2, f3 = f(-1), f(0), f(1)
+        fa, fb = (f1 + f3 - 2 * f2) / 2, (f3 - f1) / 2
+        assert abs(-fb / (2 * fa) - d) < 1e-12
+
+
+# ---------------------------------------------------------------------------------------------
+# HRRP centring and cut (range_alignment.cu:371-465, common.cu:494-518)
+# ---------------------------------------------------------------------------------------------
+def test_hrrp_center_moves_target_block_to_middle():
+    # invariant: a target block whose profile is symmetric about an integer cell c leaves a
+    # symmetric set of kept indices whatever threshold the sorted-ARP rule picks, so the integer
+    # centroid is exactly c and the circular shift is exactly N/2 - c
+    M, N = 8, 1024
+    c = 219
+    S = np.zeros((M, N), complex)
+    d = np.arange(-20, 21)
+    S[:, c + d] = 3.0 + 2.0 * np.cos(np.pi * np.abs(d) / 25.0)
+    tr = O.Trace(M, N, 512)
+    out = O.hrrp_center(S, O.params(M, N, 512), tr)
+    assert tr.c.center_applied == 1
+    assert tr.c.center_shift == N // 2 - c
+    assert np.allclose(out, np.roll(S, tr.c.center_shift, axis=1))
+    # cut: first index of the |Re|+|Im| maximum of row 0, window of 512 clamped to the row
+    cut = O.cut_range_profile(out, O.params(M, N, 512), tr)
+    key = np.abs(out[0].real) + np.abs(out[0].imag)
+    expect = min(max(int(np.argmax(key)) - 256, 0), N - 512)
+    assert tr.c.cut_offset == expect
+    assert np.array_equal(cut, out[:, expect:expect + 512])
+
+
+def test_hrrp_center_with_noise_floor():
+    # with a noise floor, bins above the threshold but with a low windowed mean are discarded
+    M, N = 8, 1024
+    rng = np.random.default_rng(10)
+    S = (rng.standard_normal((M, N)) + 1j * rng.standard_normal((M, N))) * 0.05
+    S[:, 600:660] += 5.0 * np.exp(1j * rng.uniform(0, 6.28, (M, 60)))
+    tr = O.Trace(M, N, 512)
+    O.hrrp_center(S, O.params(M, N, 512), tr)
+    assert tr.c.center_applied == 1 and tr.c.idx2_len > 0
+    assert abs(tr.c.center_shift - (N // 2 - 629)) <= 12
+
+
+# ---------------------------------------------------------------------------------------------
+# Doppler tracking and fast entropy (phase_adjustment.cu:4-41, 167-332)
+# ---------------------------------------------------------------------------------------------
+def test_doppler_tracking_recovers_pulse_phase():
+    rows, cols = 32, 40
+    rng = np.random.default_rng(11)
+    s = rng.standard_normal(cols) + 1j * rng.standard_normal(cols)
+    theta = np.cumsum(rng.uniform(-0.5, 0.5, rows))
+    X = np.exp(1j * theta)[:, None] * s[None, :]
+    comp, ph = O.doppler_tracking(X)
+    assert ph[0] == 0                                   # phase_adjustment.cu:29 keeps phi[0] = 0
+    assert np.allclose(ph[1:], np.exp(-1j * (theta[1:] - theta[0])), atol=1e-10)
+    assert np.allclose(comp[1:], np.exp(1j * theta[0]) * s[None, :].repeat(rows - 1, 0), atol=1e-10)
+
+
+@pytest.mark.parametrize("seed", [10, 13, 17])
+def test_fast_entropy_focuses_injected_phase_error(seed):
+    M, W = 64, 48
+    rng = np.random.default_rng(seed)
+    # weak clutter in every range bin, one bright point scatterer in 10 of them
+    m = np.arange(M)
+    X = np.zeros((M, W), complex)
+    bright = rng.choice(W, 10, replace=False)
+    for c in range(W):
+        X[:, c] += rng.uniform(0.2, 0.5) * np.exp(2j * np.pi * rng.integers(0, M) * m / M + 1j * rng.uniform(0, 6.28))
+        if c in bright:
+            X[:, c] += rng.uniform(4, 8) * np.exp(2j * np.pi * rng.integers(0, M) * m / M + 1j * rng.uniform(0, 6.28))
+    X *= 400.0
+    err = rng.uniform(-np.pi, np.pi, M)
+    Xe = X * np.exp(1j * err)[:, None]
+
+    def ent(img):
+        p = np.abs(img) ** 2 / np.sum(np.abs(img) ** 2)
+        return -np.sum(p[p > 0] * np.log(p[p > 0]))
+
+    tr = O.Trace(M, 2, W)
+    out = O.fast_entropy(Xe, iterations=120, trace=tr)
+    e_before = ent(np.fft.fft(Xe, axis=0))
+    e_after = ent(np.fft.fft(out, axis=0))
+    e_true = ent(np.fft.fft(X, axis=0))
+    assert e_after < e_before - 2.0
+    assert abs(e_after - e_true) < 0.01
+    # the recovered phase equals the injected error up to a constant and a linear term
+    resid = np.unwrap(np.angle(tr.phi_c * np.exp(1j * err)))
+    fit = np.polyval(np.polyfit(m, resid, 1), m)
+    assert np.abs(np.angle(np.exp(1j * (resid - fit)))).max() < 0.1
+
+
+def test_bin_selection_matches_hand_ranking():
+    # tiny matrix: per-column energy and entropy computed by hand, selection rule of
+    # phase_adjustment.cu:186-264 applied by hand, compared with the oracle's choice
+    M, W = 16, 12
+    rng = np.random.default_rng(17)
+    X = (rng.standard_normal((M, W)) + 1j * rng.standard_normal((M, W))) * rng.uniform(1, 40, W)[None, :]
+    tr = O.Trace(M, 2, W)
+    O.fast_entropy(X, iterations=0, trace=tr)
+    mx = np.abs(X).max(axis=0)
+    thr = np.float32(1.48) * np.float32(np.float32(np.sum(mx.astype(np.int32))) / np.float32(W))
+    tgt = int(np.sum(mx > thr))
+    _, phase = O.doppler_tracking(X)
+    I = np.fft.fft(phase[:, None] * X, axis=0)
+    E = np.sum(np.abs(I) ** 2, axis=0)
+    pr = np.abs(I) ** 2 / E
+    H = -np.sum(np.where(pr > 0, pr * np.log(np.where(pr > 0, pr, 1)), 0), axis=0)
+    order = sorted(range(W), key=lambda c: (-E[c], c))[:tgt]
+    order = sorted(order, key=lambda c: (H[c], order.index(c)))
+    U = O.next_pow2(tgt // 2)
+    sel = sorted((order + [0] * U)[:U])
+    assert tr.c.tgt_num == tgt and tr.c.unit2_num == U
+    assert list(tr.selected) == sel
+    assert np.allclose(tr.pa_colE[:W], E) and np.allclose(tr.pa_colH[:W], H)
+
+
+# ---------------------------------------------------------------------------------------------
+# whole chain
+# ---------------------------------------------------------------------------------------------
+def _point_scene(xs, ys, amps, M=256, N=4096, omega=0.05, prf=400.0, phase_err=None):
+    cfg = synth.preset(2)
+    h = synth.radar_header(cfg)
+    p = O.params(M, N, 512, h["band_width"], h["fc"], h["Tp"], h["Fs"])
+    c = 3e8
+    K = cfg.band_width / cfg.tp
+    fhi = cfg.fc + cfg.band_width / 2
+    t = np.arange(M) / prf
+    th = omega * (t - M / prf / 2)
+    n = np.arange(N)
+    e = np.zeros((M, N), complex)
+    for x, y, a in zip(xs, ys, amps):
+        dR = y * np.cos(th) - x * np.sin(th)
+        e += a * np.exp(1j * (-(4 * np.pi / c) * fhi * dR[:, None] + (4 * np.pi * K / c) * dR[:, None] * n / cfg.fs))
+    if phase_err is not None:
+        e *= np.exp(1j * phase_err)[:, None]
+    return (1000 * e).astype(np.complex64), np.zeros(M), p
+
+
+def test_mtrc_reduces_entropy_on_rotating_scene():
+    e, v, p = _point_scene([0, 25, -25, 12], [0, 0, 4, -5], [3, 1, 1, 1])
+    trs = []
+    for mt in (0, 1):
+        tr = O.Trace(256, 4096, 512)
+        O.isar_main(e, v, p, O.options(if_mtrc=mt, if_hpc=0, quirk_q1=0), tr)
+        trs.append(tr.c.entropy)
+    assert trs[1] < trs[0] - 0.1
+
+
+def test_point_targets_focus_to_separate_bins():
+    e, v, p = _point_scene([0, 20], [0, 6], [3, 2], phase_err=np.random.default_rng(5).uniform(-np.pi, np.pi, 256))
+    img = O.isar_main(e, v, p, O.options(if_hpc=0))
+    k = np.argsort(img.ravel())[::-1]
+    r0, c0 = divmod(int(k[0]), 512)
+    # the two strongest, well separated peaks: range separation of 6 m = 40 cells
+    others = [divmod(int(i), 512) for i in k[1:200]]
+    far = [(r, cc) for r, cc in others if abs(cc - c0) > 20]
+    assert far, "second target not resolved"
+    r1, c1 = far[0]
+    assert abs(abs(c1 - c0) - 40) <= 2
