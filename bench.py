@@ -37,8 +37,8 @@ def stage_bytes(stage: str, M: int, N: int, W: int, U: float, frames: int, launc
         "ra_upd": (M / 2) * (8.0 * N + 4.0 * N),
         # group energies read (2 groups per pair) and the first one written back: ~ M/2 rows of 4N
         "ra_corr": (M / 2) * 4.0 * N * 1.5,
-        "ra_init": M * (8.0 * N + 4.0 * N),
-        "ra_final": M * (8.0 * N + 8.0 * N),
+        # level 0: every echo row read (8N), its spectrum stored (8N), one energy row per pair (4N)
+        "ra_pair0": M * (8.0 * N + 8.0 * N) + (M / 2) * 4.0 * N,
         "autofocus": 8.0 * M * W + 8.0 * M * U * 2,
         "mtrc_czt": 2 * 8.0 * M * W,
         "mtrc_rows": 2 * 8.0 * M * W,

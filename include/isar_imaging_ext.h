@@ -109,11 +109,11 @@ ISAR_API int isar_fft_host(int n, int dir, const float* h_in, float* h_out, int 
 
 /* Kernel families that can be probed with HIP events (isar_probe_enable). */
 enum isar_stage {
-    ISAR_STAGE_IFDS_PRE = 0, ISAR_STAGE_RA_INIT = 1, ISAR_STAGE_RA_CORR = 2, ISAR_STAGE_RA_UPD = 3,
-    ISAR_STAGE_RA_FINAL = 4, ISAR_STAGE_CENTER = 5, ISAR_STAGE_CUT = 6, ISAR_STAGE_PA_COLFFT = 7,
-    ISAR_STAGE_PA_SELECT = 8, ISAR_STAGE_AUTOFOCUS = 9, ISAR_STAGE_MTRC_ROWS = 10, ISAR_STAGE_MTRC_CZT = 11,
-    ISAR_STAGE_COL_WINDOW = 12, ISAR_STAGE_MTRC_IMAGE = 13, ISAR_STAGE_POST_NOMTRC = 14,
-    ISAR_STAGE_COUNT = 15
+    ISAR_STAGE_IFDS_PRE = 0, ISAR_STAGE_RA_PAIR0 = 1, ISAR_STAGE_RA_CORR = 2, ISAR_STAGE_RA_UPD = 3,
+    ISAR_STAGE_CENTER = 4, ISAR_STAGE_CUT = 5, ISAR_STAGE_PA_COLFFT = 6, ISAR_STAGE_PA_SELECT = 7,
+    ISAR_STAGE_AUTOFOCUS = 8, ISAR_STAGE_MTRC_ROWS = 9, ISAR_STAGE_MTRC_CZT = 10, ISAR_STAGE_COL_WINDOW = 11,
+    ISAR_STAGE_MTRC_IMAGE = 12, ISAR_STAGE_POST_NOMTRC = 13,
+    ISAR_STAGE_COUNT = 14
 };
 
 /* Record a HIP event pair around every launch of kernel family `stage` (on the run stream) in

@@ -48,9 +48,9 @@ class _Trace(C.Structure):
 
 
 # kernel families (enum isar_stage in include/isar_imaging_ext.h)
-STAGES = {"ifds_pre": 0, "ra_init": 1, "ra_corr": 2, "ra_upd": 3, "ra_final": 4, "center": 5, "cut": 6,
-          "pa_colfft": 7, "pa_select": 8, "autofocus": 9, "mtrc_rows": 10, "mtrc_czt": 11, "col_window": 12,
-          "mtrc_image": 13, "post_nomtrc": 14}
+STAGES = {"ifds_pre": 0, "ra_pair0": 1, "ra_corr": 2, "ra_upd": 3, "center": 4, "cut": 5, "pa_colfft": 6,
+          "pa_select": 7, "autofocus": 8, "mtrc_rows": 9, "mtrc_czt": 10, "col_window": 11, "mtrc_image": 12,
+          "post_nomtrc": 13}
 
 _lib = None
 

@@ -47,6 +47,31 @@ __device__ __forceinline__ float2 expjpi(float t) {
     return make_float2(c, s);
 }
 
+// exp(j pi t) for t in [-2, 2] given exactly in FP32 (FFT twiddle arguments): octant reduction
+// t = q/4 + r, |r| <= 1/8, then Taylor polynomials of sin(pi r) (degree 9) and cos(pi r)
+// (degree 10), whose truncation errors (< 3e-11) are far below FP32 rounding, and a rotation by
+// exp(j pi q / 4).
+__device__ __forceinline__ float2 expjpi_twiddle(float t) {
+    const float qf = rintf(4.0f * t);
+    const int q = static_cast<int>(qf) & 7;
+    const float r = t - 0.25f * qf;   // exact for dyadic t
+    const float x = 3.14159265358979323846f * r;
+    const float x2 = x * x;
+    const float sn = x * (1.0f + x2 * (-1.6666667163e-1f + x2 * (8.3333337680e-3f + x2 * (-1.9841270114e-4f + x2 * 2.7557319224e-6f))));
+    const float cs = 1.0f + x2 * (-0.5f + x2 * (4.1666667908e-2f + x2 * (-1.3888889225e-3f + x2 * (2.4801587642e-5f - x2 * 2.7557319224e-7f))));
+    // rotation by exp(j pi q / 4) with selects only: the odd octant first (times (1 + j)/sqrt 2),
+    // then the quadrant (times j^(q/2))
+    constexpr float h = 0.70710678118654752440f;
+    const bool odd = q & 1;
+    float c = odd ? h * (cs - sn) : cs;
+    float s = odd ? h * (cs + sn) : sn;
+    const bool q1 = q & 2, q2 = q & 4;
+    const float c1 = q1 ? -s : c, s1 = q1 ? c : s;
+    c = q2 ? -c1 : c1;
+    s = q2 ? -s1 : s1;
+    return make_float2(c, s);
+}
+
 // ------------------------------------------------------------------------------------------
 // reductions
 // ------------------------------------------------------------------------------------------
@@ -279,17 +304,56 @@ struct Dft<16, DIR> {
 // Between passes the data goes through an LDS buffer with one padding slot every 16 elements.
 // All threads of the workgroup must call the transform together (it uses __syncthreads).
 // ------------------------------------------------------------------------------------------
+// A copy of v that the compiler cannot see through. Index and address arithmetic derived from it
+// is recomputed where it is used instead of being kept live in registers across transforms.
+__device__ __forceinline__ int opaque(int v) {
+#ifndef ISAR_EMU
+    asm volatile("" : "+v"(v));
+#endif
+    return v;
+}
+
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int lds_padded_size(int n) { return n + (n >> 4) + 1; }
+
+// w[r] = exp(DIR j pi base r), r < R: w^1 and w^4 from expjpi_twiddle, every other power by at
+// most three complex products (relative error a few FP32 ulp).
+template <int R, int DIR>
+__device__ __forceinline__ void twiddle_powers(float2 (&w)[R], float base) {
+    w[0] = make_float2(1.f, 0.f);
+    w[1] = expjpi_twiddle(DIR * base);
+    if constexpr (R == 3) {
+        w[2] = cmul(w[1], w[1]);
+    } else if constexpr (R >= 4) {
+        w[2] = cmul(w[1], w[1]);
+        w[3] = cmul(w[2], w[1]);
+        if constexpr (R >= 8) {
+            w[4] = expjpi_twiddle(DIR * base * 4.0f);
+            w[5] = cmul(w[4], w[1]);
+            w[6] = cmul(w[4], w[2]);
+            w[7] = cmul(w[4], w[3]);
+        }
+        if constexpr (R >= 16) {
+            const float2 w8 = cmul(w[4], w[4]);
+            w[8] = w8;
+#pragma unroll
+            for (int r = 9; r < 16; ++r) w[r] = cmul(w8, w[r - 8]);
+        }
+    }
+}
 
 template <int N, int E, int R, int DIR, int Ns>
 __device__ __forceinline__ void stockham_compute(float2 (&x)[E], int tid) {
     constexpr int T = N / E;
     constexpr int B = E / R;
     static_assert(E % R == 0, "radix must divide the elements per thread");
+    // The twiddles depend only on the thread index, so the compiler would otherwise compute
+    // those of every pass (and of every transform in a loop over rows) once and keep them live,
+    // which costs far more registers than recomputing them here.
+    const int t = Ns > 1 ? opaque(tid) : tid;
 #pragma unroll
     for (int b = 0; b < B; ++b) {
-        const int j = tid + b * T;
+        const int j = t + b * T;
         float2* v = &x[b * R];
         if constexpr (Ns > 1) {
             const int k = j % Ns;
@@ -297,14 +361,7 @@ __device__ __forceinline__ void stockham_compute(float2 (&x)[E], int tid) {
             // powers up to 8 come straight from sincospi, the rest are one or two products.
             const float base = static_cast<float>(2 * k) / static_cast<float>(Ns * R);
             float2 w[R];
-            w[0] = make_float2(1.f, 0.f);
-            w[1] = expjpi(DIR * base);
-#pragma unroll
-            for (int r = 2; r < R; ++r) {
-                if ((r & 1) == 0 && r <= 8) w[r] = expjpi(DIR * base * static_cast<float>(r));
-                else if (r < 8) w[r] = cmul(w[r - 1], w[1]);
-                else w[r] = cmul(w[8 % R], w[(r - 8) % R]);   // reached only for R = 16
-            }
+            twiddle_powers<R, DIR>(w, base);
 #pragma unroll
             for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
         }

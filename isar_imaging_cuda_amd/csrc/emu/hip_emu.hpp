@@ -100,6 +100,7 @@ using std::floor;
 using std::fabs;
 inline float sqrtf(float x) { return std::sqrt(x); }
 inline float fabsf(float x) { return std::fabs(x); }
+inline float rintf(float x) { return std::rint(x); }
 inline float fmaxf(float a, float b) { return std::fmax(a, b); }
 inline float fminf(float a, float b) { return std::fmin(a, b); }
 inline float logf(float x) { return std::log(x); }

@@ -81,7 +81,7 @@ int validate_params(const isar_radar_params& p) {
 size_t workspace_bytes(const isar_radar_params& p, int K) {
     const size_t M = p.echo_num, N = p.range_num, W = p.range_num_cut;
     const size_t F = next_pow2_ref(2 * p.echo_num - 1);
-    const size_t nchunks = (M + kRaFinalRows - 1) / kRaFinalRows;
+    const size_t nchunks = M / 2;
     const size_t nbands = (M + kCutBandRows - 1) / kCutBandRows;
     size_t per = 0;
     per += M * N * sizeof(float2);              // pre / st
@@ -147,7 +147,7 @@ int Context::init(const isar_radar_params& params, int frames) {
     max_frames = frames;
     const size_t K = frames, M = p.echo_num, N = p.range_num, W = p.range_num_cut;
     F = next_pow2_ref(2 * p.echo_num - 1);
-    const size_t nchunks = (M + kRaFinalRows - 1) / kRaFinalRows;
+    const size_t nchunks = M / 2;   // final-row chunks are indexed by head row / 2
     const size_t nbands = (M + kCutBandRows - 1) / kCutBandRows;
     bool ok = true;
     auto alloc = [&](size_t bytes) -> void* {
@@ -229,16 +229,15 @@ int Context::run(int frames, const void* echo, const double* velocity, const voi
     // a family launches at most log2(M) + 1 times per run; drain the ring only when it could overflow
     if (probe_stage >= 0 && probe_used + 16 > static_cast<int>(probe_events.size()) / 2) probe_collect();
     if (J.data_type == 1) probed(ISAR_STAGE_IFDS_PRE, s, [&] { launch_ifds_pre(J, s); });
-    probed(ISAR_STAGE_RA_INIT, s, [&] { launch_ra_init(J, s); });
-    int pair_base = 0;
-    for (int st = 1; st < M; st *= 2) {
+    probed(ISAR_STAGE_RA_PAIR0, s, [&] { launch_ra_pair0(J, s); });
+    int pair_base = M / 2;
+    for (int st = 2; st < M; st *= 2) {
         const int ngroups = (M + st - 1) / st, npairs = ngroups / 2, unpaired = ngroups & 1;
         probed(ISAR_STAGE_RA_CORR, s, [&] { launch_ra_corr(J, st, pair_base, npairs + unpaired, s); });
         const int R = std::min(st, kRaChunkRows), cpp = (st + R - 1) / R;
         if (npairs > 0) probed(ISAR_STAGE_RA_UPD, s, [&] { launch_ra_upd(J, st, pair_base, npairs * cpp, s); });
         pair_base += npairs;
     }
-    probed(ISAR_STAGE_RA_FINAL, s, [&] { launch_ra_final(J, s); });
     probed(ISAR_STAGE_CENTER, s, [&] { launch_center(J, s); });
     probed(ISAR_STAGE_CUT, s, [&] { launch_cut(J, s); });
     probed(ISAR_STAGE_PA_COLFFT, s, [&] { launch_pa_colfft(J, s); });

@@ -39,17 +39,26 @@ __global__ __launch_bounds__(kCenterThreads) void k_center(Job J) {
     float* srt = sm + N;
     float* redv = srt + N;
     int* redi = reinterpret_cast<int*>(redv + 32);
-    const int nchunks = (M + kRaFinalRows - 1) / kRaFinalRows;
+    // column sums of |S| and max |S| arrive as partials per chunk of rows that finished moving:
+    // rows 0-1 (head 0) and, at every merge stride s >= 2, chunks of min(s, kRaChunkRows) rows
+    // starting at s (the second group of pair 0). Partials are stored at head row / 2.
+    const float* ap = J.arp_part + static_cast<size_t>(f) * (M / 2) * N;
+    const float* mp = J.max_part + static_cast<size_t>(f) * (M / 2);
 
     // H = |S| / max|S| ; ARP = column sums / M  (range_alignment.cu:381-399, all M rows)
     float gm = 0.0f;
-    for (int c = tid; c < nchunks; c += NT) gm = fmaxf(gm, J.max_part[static_cast<size_t>(f) * nchunks + c]);
+    if (tid == 0) {
+        gm = mp[0];
+        for (int s = 2; s < M; s *= 2)
+            for (int h = s; h < min(2 * s, M); h += min(s, kRaChunkRows)) gm = fmaxf(gm, mp[h / 2]);
+    }
     gm = block_max<NT>(gm, redv);
     const float inv = gm > 0.0f ? 1.0f / gm : 0.0f;
     const float invM = 1.0f / static_cast<float>(M);
     for (int n = tid; n < N; n += NT) {
-        float acc = 0.0f;
-        for (int c = 0; c < nchunks; ++c) acc += J.arp_part[(static_cast<size_t>(f) * nchunks + c) * N + n];
+        float acc = ap[n];
+        for (int s = 2; s < M; s *= 2)
+            for (int h = s; h < min(2 * s, M); h += min(s, kRaChunkRows)) acc += ap[static_cast<size_t>(h / 2) * N + n];
         const float v = acc * inv * invM;
         arp[n] = v;
         srt[n] = v;

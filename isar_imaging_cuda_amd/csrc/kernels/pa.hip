@@ -160,20 +160,23 @@ __global__ __launch_bounds__(W) void k_pa_select(Job J) {
 // Fast minimum-entropy autofocus (phase_adjustment.cu:270-324): one workgroup per frame; the
 // selected bins are gathered once into a column-major copy, then J.iterations fixed-point
 // updates phi_m <- unit(sum_u conj(S[m,u]) IFFT(G |G|)[m,u] / M), G = FFT(diag(phi) S).
+// CF columns are transformed at once (TPF threads each); when all U selected bins fit in one
+// chunk they stay in registers for every iteration.
 template <int M>
 struct AfCfg {
     static constexpr int E = ColCfg<M>::E;
     static constexpr int TPF = ColCfg<M>::TPF;
-    static constexpr int NT = M <= 256 ? 256 : 512;
-    static constexpr int CF = NT / TPF;          // columns per chunk
-    static constexpr int RPT = (M + NT - 1) / NT;   // accumulator rows per thread
+    static constexpr int NT = M <= 384 ? 1024 : 512;
+    static constexpr int CF = NT / TPF;                       // columns per chunk
+    static constexpr int RPT = (M + NT - 1) / NT;             // rows per thread when M > NT
+    static constexpr int Q = M <= NT ? (NT / M >= 16 ? 16 : NT / M >= 8 ? 8 : NT / M >= 4 ? 4 : NT / M >= 2 ? 2 : 1) : 1;
 };
 
 template <int M>
 __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     using A = AfCfg<M>;
     using C = ColCfg<M>;
-    constexpr int NT = A::NT, E = A::E, TPF = A::TPF, CF = A::CF, RPT = A::RPT;
+    constexpr int NT = A::NT, E = A::E, TPF = A::TPF, CF = A::CF, RPT = A::RPT, Q = A::Q;
     HIP_DYNAMIC_SHARED(float2, sm);
     float2* phi = sm;                                     // M
     float2* rdop = phi + M;                               // M
@@ -212,21 +215,23 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     const int g = tid / TPF, lane = tid % TPF;
     float2* buf = bufs + g * lds_padded_size(M);
     const float invM = 1.0f / static_cast<float>(M);
+    const bool resident = U <= CF;   // one chunk: the columns stay in registers
+    float2 s[E];
     for (int it = 0; it < J.iterations; ++it) {
-        float2 acc[RPT];   // rows tid + i * NT
+        float2 acc[RPT];
 #pragma unroll
         for (int i = 0; i < RPT; ++i) acc[i] = make_float2(0.0f, 0.0f);
         for (int c0 = 0; c0 < U; c0 += CF) {
             const int u = c0 + g;
             const bool valid = u < U;
             const float2* col = su + static_cast<size_t>(valid ? u : 0) * M;
-            float2 s[E], x[E];
+            if (!resident || it == 0) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                const int pos = C::in(lane, e);
-                s[e] = valid ? col[pos] : make_float2(0.0f, 0.0f);
-                x[e] = cmul(phi[pos], s[e]);
+                for (int e = 0; e < E; ++e) s[e] = valid ? col[C::in(lane, e)] : make_float2(0.0f, 0.0f);
             }
+            float2 x[E];
+#pragma unroll
+            for (int e = 0; e < E; ++e) x[e] = cmul(phi[C::in(lane, e)], s[e]);
             fft<M, E, -1>(x, buf, lane);
 #pragma unroll
             for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabsf_(x[e]));
@@ -240,23 +245,41 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
             }
             __syncthreads();
             const int ncol = min(CF, U - c0);
+            if constexpr (Q > 1) {
+                // Q lanes per row, fixed summation order
+                const int m = tid / Q, q = tid % Q;
+                float2 t = make_float2(0.0f, 0.0f);
+                if (m < M)
+                    for (int k = q; k < ncol; k += Q) t = cadd(t, bufs[k * lds_padded_size(M) + m]);
+                t = group_sum2<Q>(t);
+                if (m < M) acc[0] = cadd(acc[0], t);
+            } else {
 #pragma unroll
-            for (int i = 0; i < RPT; ++i) {
-                const int m = tid + i * NT;
-                if (m < M) {
-                    float2 t = make_float2(0.0f, 0.0f);
-                    for (int k = 0; k < ncol; ++k) t = cadd(t, bufs[k * lds_padded_size(M) + m]);
-                    acc[i] = cadd(acc[i], t);
+                for (int i = 0; i < RPT; ++i) {
+                    const int m = tid + i * NT;
+                    if (m < M) {
+                        float2 t = make_float2(0.0f, 0.0f);
+                        for (int k = 0; k < ncol; ++k) t = cadd(t, bufs[k * lds_padded_size(M) + m]);
+                        acc[i] = cadd(acc[i], t);
+                    }
                 }
             }
             __syncthreads();
         }
+        if constexpr (Q > 1) {
+            const int m = tid / Q;
+            if (m < M && tid % Q == 0) {
+                const float a = cabsf_(acc[0]);
+                phi[m] = a > 0.0f ? cscale(acc[0], 1.0f / a) : make_float2(0.0f, 0.0f);
+            }
+        } else {
 #pragma unroll
-        for (int i = 0; i < RPT; ++i) {
-            const int m = tid + i * NT;
-            if (m < M) {
-                const float a = cabsf_(acc[i]);
-                phi[m] = a > 0.0f ? cscale(acc[i], 1.0f / a) : make_float2(0.0f, 0.0f);
+            for (int i = 0; i < RPT; ++i) {
+                const int m = tid + i * NT;
+                if (m < M) {
+                    const float a = cabsf_(acc[i]);
+                    phi[m] = a > 0.0f ? cscale(acc[i], 1.0f / a) : make_float2(0.0f, 0.0f);
+                }
             }
         }
         __syncthreads();

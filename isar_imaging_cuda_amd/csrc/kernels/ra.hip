@@ -1,10 +1,13 @@
 // ra.hip — range alignment (range_alignment.cu:4-201) for gfx950.
 //
-// The reference inverse-transforms all M rows at every merge level. Here each row's spectrum is
-// re-read from the raw echo with high-speed compensation, Hamming window, (-1)^n centring and
-// its cumulative lag ramp applied on the fly, and only rows whose lag changed are transformed
-// again. Group energies sum_{rows} |IFFT(row)|^2 / N^2 are kept per group head row and merged
-// level by level:  E^{l+1}_g = E^l_{2g} + E~^l_{2g+1}  (E~ = second group after its shift).
+// The reference inverse-transforms all M rows at every merge level and once more at the end.
+// Here each row's spectrum S0 (HPC, Hamming window and (-1)^n centring applied) is formed once
+// and stored; a row is transformed again only when its cumulative lag changes, by re-reading S0
+// and applying its lag ramp on the fly. Group energies sum_{rows} |IFFT(row)|^2 / N^2 are kept per
+// group head row and merged level by level, E^{l+1}_g = E^l_{2g} + E~^l_{2g+1} (E~ = the second
+// group after its shift). Row m moves for the last time at level floor(log2 m) (it moves at level
+// l exactly when bit l of m is set), so its aligned profile is written then, and no final pass
+// over all rows is needed.
 #include "../dev_common.hpp"
 #include "../launchers.hpp"
 
@@ -21,24 +24,23 @@ __device__ __forceinline__ int out_pos(int tid, int e) {
     return fft_out_pos<N, kE, FftPlan<N, kE>::R[FftPlan<N, kE>::NP - 1]>(tid, e);
 }
 
-// Spectrum of row m of frame f with HPC (FP64 phase), window, centring and the lag ramp
-// exp(-j 2 pi n lag / N) applied, in the first-pass layout.
+// Spectrum of row m of frame f with HPC (FP64 phase), window and (-1)^n centring applied, in
+// the first-pass layout (range_alignment.cu:13-18, high_speed_compensation.cu:4-54).
 template <int N>
-__device__ __forceinline__ void load_row(float2 (&x)[kE], const Job& J, int f, int m, double lag, int tid) {
+__device__ __forceinline__ void load_row_raw(float2 (&x)[kE], const Job& J, int f, int m, int tid_) {
+    const int tid = opaque(tid_);
     const size_t off = (static_cast<size_t>(f) * J.M + m) * N;
     const float2* row = (J.data_type == 1 ? J.pre : J.echo) + off;
     const bool hpc = J.if_hpc && J.data_type == 2;
     const double vk = hpc ? J.hpc_k * J.velocity[static_cast<size_t>(f) * J.M + m] : 0.0;
     const bool hrrp_window = J.if_hrrp && J.data_type == 2;
-    const double lk = -2.0 * lag / N;
-    const bool phase = (vk != 0.0) || (lag != 0.0);
 #pragma unroll
     for (int e = 0; e < kE; ++e) {
         const int n = in_pos<N>(tid, e);
         float2 v = row[n];
-        if (phase) {
+        if (hpc) {
             const double dn = static_cast<double>(n);
-            v = cmul(v, expjpi_reduced(vk * dn * dn + lk * dn));
+            v = cmul(v, expjpi_reduced(vk * dn * dn));
         }
         float w = hamming(n, N);
         if (hrrp_window) w *= w;
@@ -47,14 +49,49 @@ __device__ __forceinline__ void load_row(float2 (&x)[kE], const Job& J, int f, i
     }
 }
 
-// Energy profile of group g at merge stride s (level l = log2 s), in the first-pass layout.
+// Multiply a first-pass-layout row by the lag ramp exp(-j 2 pi n lag / N). With n = tid + e N/16
+// the ramp factors into exp(j pi b tid) * exp(j pi b N/16)^e (b = -2 lag / N); both factors and
+// the powers 4, 8, 12 are reduced in FP64 before sincos.
+template <int N>
+__device__ __forceinline__ void apply_ramp(float2 (&x)[kE], double lag, int tid) {
+    static_assert(FftPlan<N, kE>::R[0] == 16, "factorised ramp assumes a radix-16 first pass");
+    if (lag == 0.0) return;
+    const double b = -2.0 * lag / N;
+    const double bs = b * (N / 16);
+    const float2 w0 = expjpi_reduced(b * tid);
+    float2 p[4], q[4];
+    p[0] = make_float2(1.0f, 0.0f);
+    p[1] = expjpi_reduced(bs);
+    p[2] = expjpi_reduced(2.0 * bs);
+    p[3] = cmul(p[2], p[1]);
+    q[0] = w0;
+    q[1] = cmul(w0, expjpi_reduced(4.0 * bs));
+    q[2] = cmul(w0, expjpi_reduced(8.0 * bs));
+    q[3] = cmul(w0, expjpi_reduced(12.0 * bs));
+#pragma unroll
+    for (int e = 0; e < kE; ++e) x[e] = cmul(x[e], cmul(q[e >> 2], p[e & 3]));
+}
+
+// Stored spectrum S0 of row m times its lag ramp.
+template <int N>
+__device__ __forceinline__ void load_row_shifted(float2 (&x)[kE], const Job& J, int f, int m, double lag, int tid_) {
+    const int tid = opaque(tid_);
+    const float2* row = J.pre + (static_cast<size_t>(f) * J.M + m) * N;
+#pragma unroll
+    for (int e = 0; e < kE; ++e) x[e] = row[in_pos<N>(tid, e)];
+    apply_ramp<N>(x, lag, tid);
+}
+
+// Energy profile of group g at merge stride s >= 2, in the first-pass layout. Level 1 (s = 2)
+// energies are complete (written by the fused level-0 pass); from level 2 on, the shifted second
+// subgroup of the previous level is added from its row-chunk partial sums.
 template <int N>
 __device__ __forceinline__ void group_energy(float (&a)[kE], const Job& J, int f, int g, int s, int tid) {
     const int M = J.M;
     const float* base = J.egrp + (static_cast<size_t>(f) * M + static_cast<size_t>(g) * s) * N;
 #pragma unroll
     for (int e = 0; e < kE; ++e) a[e] = base[in_pos<N>(tid, e)];
-    if (s > 1) {
+    if (s > 2) {
         const int s2 = s / 2;
         const int r0 = (2 * g + 1) * s2;
         if (r0 < M) {
@@ -69,61 +106,14 @@ __device__ __forceinline__ void group_energy(float (&a)[kE], const Job& J, int f
     }
 }
 
-}  // namespace
-
-// IFDS pre-stage (isar_main.cu:15-27): ifftshift, IFFT / N, ifftshift of every row.
+// Correlation of two non-negative profiles given as energies (first-pass layout): argmax of |c|
+// after ifftshift (lowest index on ties) and the parabolic lag (range_alignment.cu:45-61,
+// 108-168). The real correlation comes from one complex FFT of gA + j gB. Thread 0 records the
+// lag and argmax; every thread gets the lag.
 template <int N>
-__global__ __launch_bounds__(N / kE) void k_ifds_pre(Job J) {
-    HIP_DYNAMIC_SHARED(float2, lds);
-    const int m = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
-    const size_t off = (static_cast<size_t>(f) * J.M + m) * N;
-    float2 x[kE];
-#pragma unroll
-    for (int e = 0; e < kE; ++e) x[e] = J.echo[off + ((in_pos<N>(tid, e) + N / 2) & (N - 1))];
-    fft<N, kE, +1>(x, lds, tid);
-    const float inv = 1.0f / N;
-#pragma unroll
-    for (int e = 0; e < kE; ++e) J.pre[off + ((out_pos<N>(tid, e) + N / 2) & (N - 1))] = cscale(x[e], inv);
-}
-
-// Level-0 energies of every row (range_alignment.cu:13-18 and :38-42 at stride 1).
-template <int N>
-__global__ __launch_bounds__(N / kE) void k_ra_init(Job J) {
-    HIP_DYNAMIC_SHARED(float2, lds);
-    const int m = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
-    float2 x[kE];
-    load_row<N>(x, J, f, m, 0.0, tid);
-    fft<N, kE, +1>(x, lds, tid);
-    const float inv = 1.0f / (static_cast<float>(N) * static_cast<float>(N));
-    float* dst = J.egrp + (static_cast<size_t>(f) * J.M + m) * N;
-#pragma unroll
-    for (int e = 0; e < kE; ++e) dst[out_pos<N>(tid, e)] = cabs2(x[e]) * inv;
-    if (tid == 0) J.lagsum[static_cast<size_t>(f) * J.M + m] = 0.0;
-}
-
-// One merge level: correlation of the two group profiles of pair p, argmax of |c| after
-// ifftshift (lowest index on ties), parabolic refinement and lag (range_alignment.cu:45-61,
-// 108-168). The real correlation comes from one complex FFT of gA + j gB.
-template <int N>
-__global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base) {
-    HIP_DYNAMIC_SHARED(float2, lds);
+__device__ float correlate_pair(const float (&a)[kE], const float (&b)[kE], float2* lds, float* redv, int* redi,
+                                int tid, float* lag_out, int32_t* imax_out) {
     constexpr int NT = N / kE;
-    float* redv = reinterpret_cast<float*>(lds + lds_padded_size(N));
-    int* redi = reinterpret_cast<int*>(redv + 16);
-    const int p = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
-    const int M = J.M;
-    const int ngroups = (M + s - 1) / s, npairs = ngroups / 2;
-    const bool unpaired = (p == npairs);
-    const int gA = unpaired ? ngroups - 1 : 2 * p;
-    float a[kE], b[kE];
-    group_energy<N>(a, J, f, gA, s, tid);
-    {   // keep the first group's energy for the next level (same head row, same thread)
-        float* dst = J.egrp + (static_cast<size_t>(f) * M + static_cast<size_t>(gA) * s) * N;
-#pragma unroll
-        for (int e = 0; e < kE; ++e) dst[in_pos<N>(tid, e)] = a[e];
-    }
-    if (unpaired) return;
-    group_energy<N>(b, J, f, gA + 1, s, tid);
     float2 x[kE];
 #pragma unroll
     for (int e = 0; e < kE; ++e) x[e] = make_float2(sqrtf(a[e]), sqrtf(b[e]));
@@ -161,16 +151,164 @@ __global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base)
             if (fa != 0.0f) delta = -fb / (2.0f * fa);
             if (!(delta == delta) || fabsf(delta) > 1e30f) delta = 0.0f;
         }
-        const size_t idx = static_cast<size_t>(f) * (M - 1) + pair_base + p;
-        J.lags[idx] = static_cast<float>(bi) + delta - static_cast<float>(N) / 2.0f;
-        J.imax[idx] = bi;
+        const float lag = static_cast<float>(bi) + delta - static_cast<float>(N) / 2.0f;
+        *lag_out = lag;
+        *imax_out = bi;
+        redv[32] = lag;
+    }
+    __syncthreads();
+    const float lag = redv[32];
+    __syncthreads();
+    return lag;
+}
+
+}  // namespace
+
+// IFDS pre-stage (isar_main.cu:15-27): ifftshift, IFFT / N, ifftshift of every row.
+template <int N>
+__global__ __launch_bounds__(N / kE) void k_ifds_pre(Job J) {
+    HIP_DYNAMIC_SHARED(float2, lds);
+    const int m = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const size_t off = (static_cast<size_t>(f) * J.M + m) * N;
+    float2 x[kE];
+#pragma unroll
+    for (int e = 0; e < kE; ++e) x[e] = J.echo[off + ((in_pos<N>(tid, e) + N / 2) & (N - 1))];
+    fft<N, kE, +1>(x, lds, tid);
+    const float inv = 1.0f / N;
+#pragma unroll
+    for (int e = 0; e < kE; ++e) J.pre[off + ((out_pos<N>(tid, e) + N / 2) & (N - 1))] = cscale(x[e], inv);
+}
+
+// Merge level 0 fused with the spectrum preparation (range_alignment.cu:13-18 and :34-73 at
+// stride 1), one workgroup per row pair (2p, 2p+1): S0 of both rows (stored for the later
+// levels), both energy profiles, their correlation and lag, the shift of row 2p+1 and the
+// merged level-1 energy. Rows 0 and 1 never move again: their aligned profiles and column sums
+// are written here.
+template <int N>
+__global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
+    HIP_DYNAMIC_SHARED(float2, lds);
+    constexpr int NT = N / kE;
+    float* redv = reinterpret_cast<float*>(lds + lds_padded_size(N));
+    int* redi = reinterpret_cast<int*>(redv + 16);
+    const int p = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const int M = J.M;
+    const int ra = 2 * p, rb = 2 * p + 1;
+    const float inv2 = 1.0f / (static_cast<float>(N) * static_cast<float>(N));
+    const float invN = 1.0f / N;
+    const bool final_rows = (p == 0);
+    float2* s0a = J.pre + (static_cast<size_t>(f) * M + ra) * N;
+    float2* s0b = J.pre + (static_cast<size_t>(f) * M + rb) * N;
+    float* dst = J.egrp + (static_cast<size_t>(f) * M + ra) * N;
+    float* ap = J.arp_part + static_cast<size_t>(f) * (M / 2) * N;
+    float2 x[kE];
+    float ea[kE];
+    float mx = 0.0f;
+    // Every global location below is re-read only by the thread that wrote it, and the stores
+    // into the S0 rows (which st aliases) are ordered by the barriers inside each transform.
+    // row 2p: S0, profile, energy (written now as the first term of the merged energy)
+    load_row_raw<N>(x, J, f, ra, tid);
+    {
+        const int t = opaque(tid);
+#pragma unroll
+        for (int e = 0; e < kE; ++e) s0a[in_pos<N>(t, e)] = x[e];
+    }
+    fft<N, kE, +1>(x, lds, tid);
+    {
+        const int t = opaque(tid);
+#pragma unroll
+        for (int e = 0; e < kE; ++e) {
+            ea[e] = cabs2(x[e]) * inv2;
+            dst[out_pos<N>(t, e)] = ea[e];
+        }
+    }
+    if (final_rows) {   // row 0 is the reference of the merge tree and never moves
+        const int t = opaque(tid);
+#pragma unroll
+        for (int e = 0; e < kE; ++e) {
+            const float2 v = cscale(x[e], invN);
+            const int pos = out_pos<N>(t, e);
+            s0a[pos] = v;
+            const float av = cabsf_(v);
+            ap[pos] = av;
+            mx = fmaxf(mx, av);
+        }
+    }
+    // row 2p+1: S0 and energy
+    load_row_raw<N>(x, J, f, rb, tid);
+    {
+        const int t = opaque(tid);
+#pragma unroll
+        for (int e = 0; e < kE; ++e) s0b[in_pos<N>(t, e)] = x[e];
+    }
+    fft<N, kE, +1>(x, lds, tid);
+    // the correlation takes the energies in the first-pass layout
+#pragma unroll
+    for (int e = 0; e < kE; ++e) x[e] = make_float2(ea[e], cabs2(x[e]) * inv2);
+    relayout_out_to_in<N, kE>(x, lds, tid);
+    float ta[kE], tb[kE];
+#pragma unroll
+    for (int e = 0; e < kE; ++e) { ta[e] = x[e].x; tb[e] = x[e].y; }
+    const size_t idx = static_cast<size_t>(f) * (M - 1) + p;
+    const float lag = correlate_pair<N>(ta, tb, lds, redv, redi, tid, J.lags + idx, J.imax + idx);
+    // shift row 2p+1 by the lag and merge (range_alignment.cu:66-69)
+    load_row_shifted<N>(x, J, f, rb, static_cast<double>(lag), tid);
+    fft<N, kE, +1>(x, lds, tid);
+    {
+        const int t = opaque(tid);
+#pragma unroll
+        for (int e = 0; e < kE; ++e) dst[out_pos<N>(t, e)] += cabs2(x[e]) * inv2;
+    }
+    if (tid == 0) {
+        J.lagsum[static_cast<size_t>(f) * M + ra] = 0.0;
+        J.lagsum[static_cast<size_t>(f) * M + rb] = static_cast<double>(lag);
+    }
+    if (final_rows) {   // row 1 never moves again either
+        const int t = opaque(tid);
+#pragma unroll
+        for (int e = 0; e < kE; ++e) {
+            const float2 v = cscale(x[e], invN);
+            const int pos = out_pos<N>(t, e);
+            s0b[pos] = v;
+            const float av = cabsf_(v);
+            ap[pos] += av;
+            mx = fmaxf(mx, av);
+        }
+        mx = block_max<NT>(mx, redv);
+        if (tid == 0) J.max_part[static_cast<size_t>(f) * (M / 2)] = mx;
     }
 }
 
-// Shift the rows of the second group of pair p by the pair's lag (range_alignment.cu:66-69)
-// and accumulate their new energies; each workgroup handles up to kRaChunkRows rows.
+// One merge level s >= 2: group energies of pair p and their correlation (range_alignment.cu:
+// 45-61). The first group's energy is kept for the next level.
 template <int N>
-__global__ __launch_bounds__(N / kE) void k_ra_upd(Job J, int s, int pair_base) {
+__global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base) {
+    HIP_DYNAMIC_SHARED(float2, lds);
+    float* redv = reinterpret_cast<float*>(lds + lds_padded_size(N));
+    int* redi = reinterpret_cast<int*>(redv + 16);
+    const int p = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const int M = J.M;
+    const int ngroups = (M + s - 1) / s, npairs = ngroups / 2;
+    const bool unpaired = (p == npairs);
+    const int gA = unpaired ? ngroups - 1 : 2 * p;
+    float a[kE], b[kE];
+    group_energy<N>(a, J, f, gA, s, tid);
+    {   // keep the first group's energy for the next level (same head row, same thread)
+        float* dst = J.egrp + (static_cast<size_t>(f) * M + static_cast<size_t>(gA) * s) * N;
+#pragma unroll
+        for (int e = 0; e < kE; ++e) dst[in_pos<N>(tid, e)] = a[e];
+    }
+    if (unpaired) return;
+    group_energy<N>(b, J, f, gA + 1, s, tid);
+    const size_t idx = static_cast<size_t>(f) * (M - 1) + pair_base + p;
+    (void)correlate_pair<N>(a, b, lds, redv, redi, tid, J.lags + idx, J.imax + idx);
+}
+
+// Shift the rows of the second group of pair p by the pair's lag (range_alignment.cu:66-69)
+// and accumulate their new energies; each workgroup handles up to kRaChunkRows rows. The rows of
+// pair 0 move for the last time at this level: their aligned profiles replace their spectra and
+// their column sums feed HRRP centring.
+template <int N>
+__global__ __launch_bounds__(N / kE, 4) void k_ra_upd(Job J, int s, int pair_base) {
     HIP_DYNAMIC_SHARED(float2, lds);
     const int b = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
     const int M = J.M;
@@ -183,16 +321,40 @@ __global__ __launch_bounds__(N / kE) void k_ra_upd(Job J, int s, int pair_base) 
     if (r0 >= r1) return;
     const float lag = J.lags[static_cast<size_t>(f) * (M - 1) + pair_base + p];
     const float inv = 1.0f / (static_cast<float>(N) * static_cast<float>(N));
+    const float invN = 1.0f / N;
+    const bool final_rows = (p == 0);
     float acc[kE];
+    float mx = 0.0f;
 #pragma unroll
     for (int e = 0; e < kE; ++e) acc[e] = 0.0f;
     float2 x[kE];
+    // column sums of |profile| over the final rows of this chunk, accumulated in place (each
+    // location is read back only by the thread that wrote it)
+    float* ap = J.arp_part + (static_cast<size_t>(f) * (M / 2) + r0 / 2) * N;
     for (int m = r0; m < r1; ++m) {
         const double L = J.lagsum[static_cast<size_t>(f) * M + m] + static_cast<double>(lag);
-        load_row<N>(x, J, f, m, L, tid);
+        load_row_shifted<N>(x, J, f, m, L, tid);
         fft<N, kE, +1>(x, lds, tid);
 #pragma unroll
         for (int e = 0; e < kE; ++e) acc[e] += cabs2(x[e]) * inv;
+        if (final_rows) {   // S0 of row m was read before the transform's first barrier
+            float2* st = J.st + (static_cast<size_t>(f) * M + m) * N;
+            const int t = opaque(tid);
+#pragma unroll
+            for (int e = 0; e < kE; ++e) {
+                const float2 v = cscale(x[e], invN);
+                const int pos = out_pos<N>(t, e);
+                st[pos] = v;
+                const float av = cabsf_(v);
+                ap[pos] = (m == r0 ? 0.0f : ap[pos]) + av;
+                mx = fmaxf(mx, av);
+            }
+        }
+    }
+    if (final_rows) {
+        float* red = reinterpret_cast<float*>(lds + lds_padded_size(N));
+        mx = block_max<N / kE>(mx, red);
+        if (tid == 0) J.max_part[static_cast<size_t>(f) * (M / 2) + r0 / 2] = mx;
     }
     __syncthreads();
     if (tid < r1 - r0) J.lagsum[static_cast<size_t>(f) * M + r0 + tid] += static_cast<double>(lag);
@@ -201,49 +363,12 @@ __global__ __launch_bounds__(N / kE) void k_ra_upd(Job J, int s, int pair_base) 
     for (int e = 0; e < kE; ++e) dst[out_pos<N>(tid, e)] = acc[e];
 }
 
-// Final pass (range_alignment.cu:76-77): aligned range profiles IFFT(S) / N, plus the column
-// sums of |S| and the maximum |S| per row chunk for HRRP centring (range_alignment.cu:381-399).
-template <int N>
-__global__ __launch_bounds__(N / kE) void k_ra_final(Job J) {
-    HIP_DYNAMIC_SHARED(float2, lds);
-    constexpr int NT = N / kE;
-    float* red = reinterpret_cast<float*>(lds + lds_padded_size(N));
-    const int c = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
-    const int M = J.M;
-    const int nchunks = (M + kRaFinalRows - 1) / kRaFinalRows;
-    const int r0 = c * kRaFinalRows, r1 = min(r0 + kRaFinalRows, M);
-    const float inv = 1.0f / N;
-    float acc[kE];
-    float mx = 0.0f;
-#pragma unroll
-    for (int e = 0; e < kE; ++e) acc[e] = 0.0f;
-    float2 x[kE];
-    for (int m = r0; m < r1; ++m) {
-        load_row<N>(x, J, f, m, J.lagsum[static_cast<size_t>(f) * M + m], tid);
-        fft<N, kE, +1>(x, lds, tid);
-        float2* dst = J.st + (static_cast<size_t>(f) * M + m) * N;
-#pragma unroll
-        for (int e = 0; e < kE; ++e) {
-            const float2 v = cscale(x[e], inv);
-            dst[out_pos<N>(tid, e)] = v;
-            const float a = cabsf_(v);
-            acc[e] += a;
-            mx = fmaxf(mx, a);
-        }
-    }
-    float* ap = J.arp_part + (static_cast<size_t>(f) * nchunks + c) * N;
-#pragma unroll
-    for (int e = 0; e < kE; ++e) ap[out_pos<N>(tid, e)] = acc[e];
-    mx = block_max<NT>(mx, red);
-    if (tid == 0) J.max_part[static_cast<size_t>(f) * nchunks + c] = mx;
-}
-
 // ------------------------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------------------------
 namespace {
 template <int N>
-size_t ra_lds() { return lds_padded_size(N) * sizeof(float2) + 32 * sizeof(float); }
+size_t ra_lds() { return lds_padded_size(N) * sizeof(float2) + 64 * sizeof(float); }
 
 #define ISAR_RANGE_SWITCH(N, BODY)                     \
     switch (N) {                                       \
@@ -272,10 +397,9 @@ hipError_t configure_ra_kernels() {
     };
 #define ISAR_SET_RA(NN)                                                        \
     set(reinterpret_cast<const void*>(&k_ifds_pre<NN>), ra_lds<NN>());         \
-    set(reinterpret_cast<const void*>(&k_ra_init<NN>), ra_lds<NN>());          \
+    set(reinterpret_cast<const void*>(&k_ra_pair0<NN>), ra_lds<NN>());         \
     set(reinterpret_cast<const void*>(&k_ra_corr<NN>), ra_lds<NN>());          \
-    set(reinterpret_cast<const void*>(&k_ra_upd<NN>), ra_lds<NN>());           \
-    set(reinterpret_cast<const void*>(&k_ra_final<NN>), ra_lds<NN>());
+    set(reinterpret_cast<const void*>(&k_ra_upd<NN>), ra_lds<NN>());
     ISAR_SET_RA(1024) ISAR_SET_RA(2048) ISAR_SET_RA(4096) ISAR_SET_RA(8192) ISAR_SET_RA(16384)
 #undef ISAR_SET_RA
 #endif
@@ -285,8 +409,8 @@ hipError_t configure_ra_kernels() {
 void launch_ifds_pre(const Job& J, hipStream_t s) {
     ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH(k_ifds_pre<NN>, dim3(J.M, J.K), dim3(NN / kE), ra_lds<NN>(), s, J));
 }
-void launch_ra_init(const Job& J, hipStream_t s) {
-    ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH(k_ra_init<NN>, dim3(J.M, J.K), dim3(NN / kE), ra_lds<NN>(), s, J));
+void launch_ra_pair0(const Job& J, hipStream_t s) {
+    ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH(k_ra_pair0<NN>, dim3(J.M / 2, J.K), dim3(NN / kE), ra_lds<NN>(), s, J));
 }
 void launch_ra_corr(const Job& J, int stride, int pair_base, int nblocks, hipStream_t s) {
     ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH(k_ra_corr<NN>, dim3(nblocks, J.K), dim3(NN / kE), ra_lds<NN>(), s, J,
@@ -295,10 +419,6 @@ void launch_ra_corr(const Job& J, int stride, int pair_base, int nblocks, hipStr
 void launch_ra_upd(const Job& J, int stride, int pair_base, int nblocks, hipStream_t s) {
     ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH(k_ra_upd<NN>, dim3(nblocks, J.K), dim3(NN / kE), ra_lds<NN>(), s, J,
                                        stride, pair_base));
-}
-void launch_ra_final(const Job& J, hipStream_t s) {
-    const int nchunks = (J.M + kRaFinalRows - 1) / kRaFinalRows;
-    ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH(k_ra_final<NN>, dim3(nchunks, J.K), dim3(NN / kE), ra_lds<NN>(), s, J));
 }
 
 }  // namespace isar

@@ -16,10 +16,9 @@ void report_config_error(const char* file, size_t lds_bytes);
 
 // range alignment (range_alignment.cu:4-78)
 void launch_ifds_pre(const Job& J, hipStream_t s);
-void launch_ra_init(const Job& J, hipStream_t s);
+void launch_ra_pair0(const Job& J, hipStream_t s);
 void launch_ra_corr(const Job& J, int stride, int pair_base, int nblocks, hipStream_t s);
 void launch_ra_upd(const Job& J, int stride, int pair_base, int nblocks, hipStream_t s);
-void launch_ra_final(const Job& J, hipStream_t s);
 
 // HRRP centring, cut and range-window statistics (range_alignment.cu:371-465, common.cu:494-518)
 void launch_center(const Job& J, hipStream_t s);

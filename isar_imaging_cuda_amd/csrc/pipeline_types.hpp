@@ -15,7 +15,6 @@ namespace isar {
 
 constexpr int kRangeCut = 512;        // RANGE_NUM_CUT (common.cuh:71)
 constexpr int kRaChunkRows = 8;       // rows per workgroup in the alignment update kernel
-constexpr int kRaFinalRows = 8;       // rows per workgroup in the final alignment pass
 constexpr int kCutBandRows = 16;      // rows per workgroup when cutting the range window
 
 // Per-frame scalar decisions written by the device (one record per frame).
@@ -49,15 +48,15 @@ struct Job {
     const double* velocity;       // K x M
     const float2* phi0_in;        // K x M (optional warm start)
     // workspaces
-    float2* pre;                  // K x M x N (IFDS pre-stage output)
+    float2* pre;                  // K x M x N: IFDS pre-stage output, then the row spectra S0
     double* lagsum;               // K x M
     float* lags;                  // K x (M-1)
     int32_t* imax;                // K x (M-1)
     float* egrp;                  // K x M x N
     float* epart;                 // K x M x N
-    float2* st;                   // K x M x N (aligned range profiles)
-    float* arp_part;              // K x ceil(M/kRaFinalRows) x N
-    float* max_part;              // K x ceil(M/kRaFinalRows)
+    float2* st;                   // K x M x N (aligned range profiles; aliases pre)
+    float* arp_part;              // K x M/2 x N: column sums of |profile| per final-row chunk
+    float* max_part;              // K x M/2: max |profile| per final-row chunk
     FrameDecisions* dec;          // K
     float2* x;                    // K x M x W (cut)
     float2* rowdop;               // K x M
