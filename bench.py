@@ -4,7 +4,7 @@ One step = the full imaging chain (HPC, range alignment, centring and cut, Doppl
 120-iteration fast-entropy autofocus, MTRC and the azimuth FFT) over one batch of `--frames`
 distinct synthetic frames already resident in HBM. Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--frames 64] [--no-cpu]
+  python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--frames 256] [--probe ra_upd] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -30,27 +30,53 @@ def algorithmic_bytes_per_frame(M: int, N: int, W: int, U: float) -> float:
     return 8.0 * M * N * (L + 2) + 76.0 * M * W + 8.0 * M * U
 
 
-def stage_bytes(stage: str, M: int, N: int, W: int, U: float, frames: int, launches_per_run: int) -> float:
-    """Algorithmic HBM bytes of one launch of a kernel family (whole batch), see DESIGN.md."""
-    per_frame = {
-        # every merge level re-reads the M/2 shifted rows (8N each) and writes their energy (4N)
-        "ra_upd": (M / 2) * (8.0 * N + 4.0 * N),
-        # group energies read (2 groups per pair) and the first one written back: ~ M/2 rows of 4N
-        "ra_corr": (M / 2) * 4.0 * N * 1.5,
-        # level 0: every echo row read (8N), its spectrum stored (8N), one energy row per pair (4N)
-        "ra_pair0": M * (8.0 * N + 8.0 * N) + (M / 2) * 4.0 * N,
-        "autofocus": 8.0 * M * W + 8.0 * M * U * 2,
+def _ra_levels(M: int) -> list[int]:
+    """Merge strides s >= 2 of range alignment (level 0 is the fused pair kernel)."""
+    out, s = [], 2
+    while s < M:
+        out.append(s)
+        s *= 2
+    return out
+
+
+def stage_bytes_per_frame(stage: str, M: int, N: int, W: int, U: float) -> float:
+    """Algorithmic HBM bytes of one frame for one launch of a kernel family, averaged over the
+    launches of that family in one run (range-alignment families launch once per merge level).
+
+    ra_pair0   read every echo row (8N), store its spectrum S0 (8N), one merged energy row per
+               pair (4N), the two final profiles of rows 0-1 (8N each) and their column sums (4N)
+    ra_upd(s)  read S0 of the M/2 rows that move (8N each), one partial energy row (4N) per
+               chunk of R = min(s, 8) moved rows, final profiles of pair 0's s moved rows (8N each)
+               and their per-chunk column sums (4N)
+    ra_corr(s) per pair: two group energies read (4N each) plus the chunk partials of each
+               group's second half, the first group's energy written back (4N)
+    """
+    if stage == "ra_pair0":
+        return M * 16.0 * N + (M / 2) * 4.0 * N + 2 * 8.0 * N + 4.0 * N
+    if stage == "ra_upd":
+        per = []
+        for s in _ra_levels(M):
+            R = min(s, 8)
+            per.append((M / 2) * 8.0 * N + (M / 2 / R) * 4.0 * N + s * 8.0 * N + (s / R) * 4.0 * N)
+        return sum(per) / len(per)
+    if stage == "ra_corr":
+        per = []
+        for s in _ra_levels(M):
+            pairs = M // (2 * s)
+            parts = 0 if s == 2 else (s // 2) // min(s // 2, 8)
+            per.append(pairs * (2 * 4.0 * N + 2 * parts * 4.0 * N + 4.0 * N))
+        return sum(per) / len(per)
+    return {
+        "autofocus": 8.0 * M * U + 8.0 * M * 2,
         "mtrc_czt": 2 * 8.0 * M * W,
         "mtrc_rows": 2 * 8.0 * M * W,
         "mtrc_image": 8.0 * M * W + 4.0 * M * W,
         "pa_colfft": 8.0 * M * W,
+        "cut": 2 * 8.0 * M * W,
     }[stage]
-    if stage in ("ra_upd", "ra_corr"):
-        per_frame /= 1.0
-    return per_frame * frames
 
 
-def cpu_baseline(frames: int = 24) -> dict:
+def cpu_baseline(frames: int = 128) -> dict:
     """FP64 oracle with OpenMP over frames on this host (bounded sample of the same workload)."""
     from isar_imaging_cuda_amd import synth
     from oracle import oracle as O
@@ -70,21 +96,52 @@ def cpu_baseline(frames: int = 24) -> dict:
                       f"{dt:.1f} s wall"}
 
 
+def copy_calibration(mib: int) -> int:
+    """A plain device copy of known size, for scaling the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                      "launches": 3}))
+    return 0
+
+
+def load_traffic(kernel: str, frames: int) -> dict | None:
+    """HBM traffic per launch of `kernel` at this workload from a committed PMC summary
+    (profiles/traffic_<kernel>_256x4096.json, written from two rocprofv3 --pmc runs and scaled by
+    the same counters read on a plain copy). None when no summary exists."""
+    path = ROOT / "profiles" / f"traffic_{kernel}_256x4096.json"
+    if not path.exists():
+        return None
+    d = json.loads(path.read_text())
+    per_frame = d["hbm_bytes_per_launch"] / d["frames_per_launch"]
+    return {"bytes": per_frame * frames, "source": path.name}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--frames", type=int, default=64, help="frames per batch (one step)")
+    ap.add_argument("--frames", type=int, default=256, help="frames per batch (one step)")
     ap.add_argument("--distinct", type=int, default=64, help="distinct synthetic frames in the batch")
-    ap.add_argument("--probe", default="auto", help="kernel family timed for the roofline figure")
+    ap.add_argument("--probe", default="ra_upd", help="kernel family timed for the roofline figure")
+    ap.add_argument("--probe-steps", type=int, default=3, help="extra steps timed per launch for the roofline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-frames", type=int, default=24)
+    ap.add_argument("--cpu-frames", type=int, default=128)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the imaging path shards "
-              f"frames across GPUs with no exchange, see DESIGN.md).", file=sys.stderr)
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported: the imaging path shards "
+              f"by frames with no exchange between GPUs, and this build measures one GPU.", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
 
     import torch
 
@@ -115,30 +172,39 @@ def main() -> int:
     torch.cuda.synchronize()
     # mean number of selected bins (needed for the algorithmic byte count)
     U = float(np.mean([im.trace(k).unit2_num for k in range(min(K, D))]))
+    print(f"[bench] warm-up done: {K} frames/step, mean U = {U:.1f}", file=sys.stderr, flush=True)
 
-    probe = "autofocus" if args.probe == "auto" else args.probe
-    im.probe_enable(isar.STAGES[probe])
+    # timed region: exactly `steps` steps, nothing else on the stream
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
+    t0 = time.perf_counter()
     ev0.record(stream)
     for _ in range(args.steps):
         step()
     ev1.record(stream)
     torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - t0) * 1e3
     total_ms = ev0.elapsed_time(ev1)
-    probe_ms, probe_n = im.probe_read()
-    im.probe_enable(-1)
     ms_per_step = total_ms / args.steps
     fps = K * args.steps / (total_ms / 1e3)
 
-    launches_per_run = probe_n / max(args.steps, 1)
+    # roofline of the dominant kernel family: HIP events around each of its launches on the run
+    # stream, in separate steps after the timed region
+    probe = args.probe
+    im.probe_enable(isar.STAGES[probe])
+    for _ in range(args.probe_steps):
+        step()
+    torch.cuda.synchronize()
+    probe_ms, probe_n = im.probe_read()
+    im.probe_enable(-1)
     avg_launch_ms = probe_ms / max(probe_n, 1)
-    alg = stage_bytes(probe, M, N, W, U, K, launches_per_run)
-    achieved_gbs = alg / (avg_launch_ms / 1e3) / 1e9
+    alg = stage_bytes_per_frame(probe, M, N, W, U) * K
+    achieved_gbs = alg / (avg_launch_ms / 1e3) / 1e9 if probe_n else 0.0
+    traffic = load_traffic(probe, K)
     frame_bytes = algorithmic_bytes_per_frame(M, N, W, U)
     out = {
-        "metric": "frames/s @256x4096 cf32, % HBM roofline",
+        "metric": "frames/s @256x4096 cf32, % HBM roofline; learned vs plain at equal entropy",
         "value": round(fps, 2),
         "unit": "frames/s",
         "n_gpus": 1,
@@ -150,12 +216,16 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "c64 (fp32 complex; fp64 phase arguments)",
         "data": "synthetic",
-        "config": {"workload": "config 2: 256 pulses x 4096 range cells complex64, full plain pipeline "
-                               "(HPC, RA, centring, 120-iteration fast entropy, MTRC, azimuth FFT)",
-                   "frames_per_step": K, "distinct_frames": D, "mean_selected_bins": U},
+        "config": {"workload": "256 pulses x 4096 range cells complex64 STRETCH frames, full plain pipeline "
+                               "(HPC, range alignment, centring and cut, 120-iteration fast entropy, MTRC, "
+                               "azimuth FFT)",
+                   "frames_per_step": K, "distinct_frames": D, "mean_selected_bins": U,
+                   "wall_ms_timed_region": round(wall_ms, 3)},
         "roofline": {"bound": "hbm", "kernel": probe, "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5), "traffic": None,
-                     "avg_launch_ms": round(avg_launch_ms, 5), "launches_per_step": launches_per_run,
+                     "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
+                     "traffic": None if traffic is None else round(traffic["bytes"]),
+                     "traffic_source": None if traffic is None else traffic["source"],
+                     "avg_launch_ms": round(avg_launch_ms, 5), "launches_timed": probe_n,
                      "algorithmic_bytes_per_launch": alg},
         "pipeline_roofline": {"algorithmic_bytes_per_frame": frame_bytes,
                               "achieved_GBs": round(frame_bytes * fps / 1e9, 2),

@@ -31,14 +31,13 @@ __device__ __forceinline__ float2 cmul_i(float2 a) {
     return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
 }
 
-// exp(j pi t) for a phase given in half-turns; t is reduced in FP64 before the FP32 sincospi,
-// so arguments carrying n^2 or k*s growth keep their digits.
+__device__ __forceinline__ float2 expjpi_twiddle(float t);
+
+// exp(j pi t) for a phase given in half-turns; t is reduced to [-1, 1] in FP64 before the FP32
+// evaluation, so arguments carrying n^2 or k*s growth keep their digits.
 __device__ __forceinline__ float2 expjpi_reduced(double t) {
-    t -= 2.0 * floor(t * 0.5);                     // [0, 2)
-    if (t > 1.0) t -= 2.0;                         // (-1, 1]
-    float s, c;
-    sincospif(static_cast<float>(t), &s, &c);
-    return make_float2(c, s);
+    t -= 2.0 * rint(0.5 * t);
+    return expjpi_twiddle(static_cast<float>(t));
 }
 // exp(j pi t) for |t| small enough that FP32 holds it exactly (dyadic FFT twiddles).
 __device__ __forceinline__ float2 expjpi(float t) {
@@ -312,6 +311,21 @@ __device__ __forceinline__ int opaque(int v) {
 #endif
     return v;
 }
+// Value of v held by lane `lane` (a compile-time constant in practice) of the calling wave.
+__device__ __forceinline__ float wave_read_lane(float v, int lane) {
+#ifdef ISAR_EMU
+    return __shfl(v, lane, 64);
+#else
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+#endif
+}
+
+__device__ __forceinline__ float2 opaque(float2 v) {
+#ifndef ISAR_EMU
+    asm volatile("" : "+v"(v.x), "+v"(v.y));
+#endif
+    return v;
+}
 
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int lds_padded_size(int n) { return n + (n >> 4) + 1; }
@@ -340,6 +354,51 @@ __device__ __forceinline__ void twiddle_powers(float2 (&w)[R], float base) {
             for (int r = 9; r < 16; ++r) w[r] = cmul(w8, w[r - 8]);
         }
     }
+}
+
+// w[r] for r < R from given exp(DIR j pi base) and exp(DIR j pi 4 base).
+template <int R>
+__device__ __forceinline__ void twiddle_powers_from(float2 (&w)[R], float2 w1, float2 w4) {
+    w[0] = make_float2(1.f, 0.f);
+    w[1] = w1;
+    if constexpr (R >= 3) w[2] = cmul(w1, w1);
+    if constexpr (R >= 4) w[3] = cmul(w[2], w1);
+    if constexpr (R >= 8) {
+        w[4] = w4;
+        w[5] = cmul(w4, w1);
+        w[6] = cmul(w4, w[2]);
+        w[7] = cmul(w4, w[3]);
+    }
+    if constexpr (R >= 16) {
+        const float2 w8 = cmul(w4, w4);
+        w[8] = w8;
+#pragma unroll
+        for (int r = 9; r < 16; ++r) w[r] = cmul(w8, w[r - 8]);
+    }
+}
+
+// Twiddle bases of every pass of a transform plan for one thread, computed once and reused by
+// all transforms of a kernel: w1[p] = exp(+j pi b), w4[p] = exp(+j pi 4 b), b = 2k / (Ns R) with
+// k = tid mod Ns, for the passes that hold one radix group per thread (the others compute
+// their twiddles in place).
+template <int N, int E>
+struct FftTwiddles {
+    float2 w1[5], w4[5];
+};
+
+template <int N, int E, int DIR, int Ns, int R>
+__device__ __forceinline__ void stockham_twiddle_cached(float2* v, float2 w1, float2 w4) {
+    // the powers are formed here, at each use, rather than once per kernel and kept live
+    w1 = opaque(w1);
+    w4 = opaque(w4);
+    if constexpr (DIR < 0) {
+        w1.y = -w1.y;
+        w4.y = -w4.y;
+    }
+    float2 w[R];
+    twiddle_powers_from<R>(w, w1, w4);
+#pragma unroll
+    for (int r = 1; r < R; ++r) v[r] = cmul(v[r], w[r]);
 }
 
 template <int N, int E, int R, int DIR, int Ns>
@@ -448,11 +507,63 @@ struct FftPasses {
     }
 };
 
+// Same passes with the twiddle bases taken from a per-thread cache (one group per thread).
+template <int N, int E, int DIR, int P, int Ns>
+struct FftPassesCached {
+    using Plan = FftPlan<N, E>;
+    __device__ __forceinline__ static void run(float2 (&x)[E], float2* lds, int tid, const FftTwiddles<N, E>& tw) {
+        constexpr int R = Plan::R[P];
+        if constexpr (E == R) {   // one radix group per thread: twiddles from the cache
+            if constexpr (Ns > 1) stockham_twiddle_cached<N, E, DIR, Ns, R>(x, tw.w1[P], tw.w4[P]);
+            Dft<R, DIR>::run(x);
+        } else {
+            stockham_compute<N, E, R, DIR, Ns>(x, tid);
+        }
+        if constexpr (P + 1 < Plan::NP) {
+            constexpr int RN = Plan::R[P + 1];
+            stockham_store<N, E, R, Ns>(x, lds, tid);
+            __syncthreads();
+            stockham_load<N, E, RN>(x, lds, tid);
+            __syncthreads();
+            FftPassesCached<N, E, DIR, P + 1, Ns * R>::run(x, lds, tid, tw);
+        }
+    }
+};
+
+template <int N, int E, int P, int Ns>
+__device__ __forceinline__ void fft_twiddles_fill(FftTwiddles<N, E>& tw, int tid) {
+    using Plan = FftPlan<N, E>;
+    if constexpr (P < Plan::NP) {
+        constexpr int R = Plan::R[P];
+        if constexpr (Ns > 1 && R == E) {
+            const float base = static_cast<float>(2 * (tid % Ns)) / static_cast<float>(Ns * R);
+            tw.w1[P] = expjpi_twiddle(base);
+            tw.w4[P] = R >= 8 ? expjpi_twiddle(4.0f * base) : make_float2(1.0f, 0.0f);
+        } else {
+            tw.w1[P] = make_float2(1.0f, 0.0f);
+            tw.w4[P] = make_float2(1.0f, 0.0f);
+        }
+        fft_twiddles_fill<N, E, P + 1, Ns * R>(tw, tid);
+    }
+}
+
+template <int N, int E>
+__device__ __forceinline__ FftTwiddles<N, E> fft_twiddles(int tid) {
+    FftTwiddles<N, E> tw;
+    fft_twiddles_fill<N, E, 0, 1>(tw, tid);
+    return tw;
+}
+
 // In-register/LDS transform. On entry x[e] = in[fft_in_pos(tid, e)], on exit
 // x[e] = out[fft_out_pos(tid, e)]. lds must hold lds_padded_size(N) float2 for this transform.
 template <int N, int E, int DIR>
 __device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid) {
     FftPasses<N, E, DIR, 0, 1>::run(x, lds, tid);
+}
+// The same transform with twiddle bases from fft_twiddles<N, E>(tid).
+template <int N, int E, int DIR>
+__device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid, const FftTwiddles<N, E>& tw) {
+    FftPassesCached<N, E, DIR, 0, 1>::run(x, lds, tid, tw);
 }
 
 template <int N, int E>

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 namespace isar {
 
@@ -19,6 +20,7 @@ namespace {
 std::mutex g_err_mu;
 std::string g_last_error;
 constexpr double kLightSpeed = 3e8;   // LIGHT_SPEED (common.cuh:69)
+constexpr double kPi = 3.14159265358979323846;
 }  // namespace
 
 void set_error(const std::string& msg) {
@@ -186,6 +188,7 @@ int Context::init(const isar_radar_params& params, int frames) {
     J.czt = static_cast<float2*>(alloc(K * M * W * sizeof(float2)));
     J.colsum = static_cast<float2*>(alloc(K * W * sizeof(float2)));
     czt_table = static_cast<float2*>(alloc(W * F * sizeof(float2)));
+    float* win_range = static_cast<float*>(alloc(N * sizeof(float)));
     if (!ok) return fail(ISAR_ERR_MEMORY, "device allocation failed for " + std::to_string(frames) + " frames");
     // constants (high_speed_compensation.cu:16-18, mtrc.cu:10-12, 47)
     const double B = static_cast<double>(p.band_width), fc = static_cast<double>(p.fc), Fs = static_cast<double>(p.Fs);
@@ -193,6 +196,14 @@ int Context::init(const isar_radar_params& params, int frames) {
     J.czt_c0 = B / (2.0 * fc);
     J.czt_posa = B / (fc * (static_cast<double>(W) - 1.0));
     J.czt_table = czt_table;
+    {   // Hamming window over the range cells, evaluated in double on the host once
+        std::vector<float> w(N);
+        for (size_t n = 0; n < N; ++n)
+            w[n] = static_cast<float>(0.54 - 0.46 * std::cos(2.0 * kPi * static_cast<double>(n) / static_cast<double>(N - 1)));
+        if (hipMemcpy(win_range, w.data(), N * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(ISAR_ERR_DEVICE, "window upload failed");
+        J.win_range = win_range;
+    }
     launch_czt_table(J.M, J.W, F, J.czt_c0, J.czt_posa, czt_table, nullptr);
     hipError_t e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipGetLastError();

@@ -42,7 +42,7 @@ __device__ __forceinline__ void load_row_raw(float2 (&x)[kE], const Job& J, int 
             const double dn = static_cast<double>(n);
             v = cmul(v, expjpi_reduced(vk * dn * dn));
         }
-        float w = hamming(n, N);
+        float w = J.win_range[n];
         if (hrrp_window) w *= w;
         if (n & 1) w = -w;
         x[e] = cscale(v, w);
@@ -50,26 +50,21 @@ __device__ __forceinline__ void load_row_raw(float2 (&x)[kE], const Job& J, int 
 }
 
 // Multiply a first-pass-layout row by the lag ramp exp(-j 2 pi n lag / N). With n = tid + e N/16
-// the ramp factors into exp(j pi b tid) * exp(j pi b N/16)^e (b = -2 lag / N); both factors and
-// the powers 4, 8, 12 are reduced in FP64 before sincos.
+// the ramp factors into exp(j pi b tid) * u_e, u_e = exp(j pi b e N/16), b = -2 lag / N. The u_e
+// are the same for every thread: lane e of each wave evaluates u_e and the other lanes read it.
+// Both arguments are reduced in FP64.
 template <int N>
 __device__ __forceinline__ void apply_ramp(float2 (&x)[kE], double lag, int tid) {
     static_assert(FftPlan<N, kE>::R[0] == 16, "factorised ramp assumes a radix-16 first pass");
     if (lag == 0.0) return;
     const double b = -2.0 * lag / N;
-    const double bs = b * (N / 16);
     const float2 w0 = expjpi_reduced(b * tid);
-    float2 p[4], q[4];
-    p[0] = make_float2(1.0f, 0.0f);
-    p[1] = expjpi_reduced(bs);
-    p[2] = expjpi_reduced(2.0 * bs);
-    p[3] = cmul(p[2], p[1]);
-    q[0] = w0;
-    q[1] = cmul(w0, expjpi_reduced(4.0 * bs));
-    q[2] = cmul(w0, expjpi_reduced(8.0 * bs));
-    q[3] = cmul(w0, expjpi_reduced(12.0 * bs));
+    const float2 u = expjpi_reduced(b * static_cast<double>((N / 16) * (tid & 15)));
 #pragma unroll
-    for (int e = 0; e < kE; ++e) x[e] = cmul(x[e], cmul(q[e >> 2], p[e & 3]));
+    for (int e = 0; e < kE; ++e) {
+        const float2 ue = make_float2(wave_read_lane(u.x, e), wave_read_lane(u.y, e));
+        x[e] = cmul(cmul(x[e], ue), w0);
+    }
 }
 
 // Stored spectrum S0 of row m times its lag ramp.
@@ -112,12 +107,12 @@ __device__ __forceinline__ void group_energy(float (&a)[kE], const Job& J, int f
 // lag and argmax; every thread gets the lag.
 template <int N>
 __device__ float correlate_pair(const float (&a)[kE], const float (&b)[kE], float2* lds, float* redv, int* redi,
-                                int tid, float* lag_out, int32_t* imax_out) {
+                                int tid, const FftTwiddles<N, kE>& tw, float* lag_out, int32_t* imax_out) {
     constexpr int NT = N / kE;
     float2 x[kE];
 #pragma unroll
     for (int e = 0; e < kE; ++e) x[e] = make_float2(sqrtf(a[e]), sqrtf(b[e]));
-    fft<N, kE, -1>(x, lds, tid);
+    fft<N, kE, -1>(x, lds, tid, tw);
 #pragma unroll
     for (int e = 0; e < kE; ++e) lds[lds_pad(out_pos<N>(tid, e))] = x[e];
     __syncthreads();
@@ -130,7 +125,7 @@ __device__ float correlate_pair(const float (&a)[kE], const float (&b)[kE], floa
         x[e] = cmulc(ga, gb);
     }
     __syncthreads();
-    fft<N, kE, +1>(x, lds, tid);
+    fft<N, kE, +1>(x, lds, tid, tw);
     float best = -1.0f;
     int bi = 0;
 #pragma unroll
@@ -200,6 +195,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
     float2* s0b = J.pre + (static_cast<size_t>(f) * M + rb) * N;
     float* dst = J.egrp + (static_cast<size_t>(f) * M + ra) * N;
     float* ap = J.arp_part + static_cast<size_t>(f) * (M / 2) * N;
+    const FftTwiddles<N, kE> tw = fft_twiddles<N, kE>(tid);
     float2 x[kE];
     float ea[kE];
     float mx = 0.0f;
@@ -212,7 +208,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
 #pragma unroll
         for (int e = 0; e < kE; ++e) s0a[in_pos<N>(t, e)] = x[e];
     }
-    fft<N, kE, +1>(x, lds, tid);
+    fft<N, kE, +1>(x, lds, tid, tw);
     {
         const int t = opaque(tid);
 #pragma unroll
@@ -240,7 +236,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
 #pragma unroll
         for (int e = 0; e < kE; ++e) s0b[in_pos<N>(t, e)] = x[e];
     }
-    fft<N, kE, +1>(x, lds, tid);
+    fft<N, kE, +1>(x, lds, tid, tw);
     // the correlation takes the energies in the first-pass layout
 #pragma unroll
     for (int e = 0; e < kE; ++e) x[e] = make_float2(ea[e], cabs2(x[e]) * inv2);
@@ -249,10 +245,10 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
 #pragma unroll
     for (int e = 0; e < kE; ++e) { ta[e] = x[e].x; tb[e] = x[e].y; }
     const size_t idx = static_cast<size_t>(f) * (M - 1) + p;
-    const float lag = correlate_pair<N>(ta, tb, lds, redv, redi, tid, J.lags + idx, J.imax + idx);
+    const float lag = correlate_pair<N>(ta, tb, lds, redv, redi, tid, tw, J.lags + idx, J.imax + idx);
     // shift row 2p+1 by the lag and merge (range_alignment.cu:66-69)
     load_row_shifted<N>(x, J, f, rb, static_cast<double>(lag), tid);
-    fft<N, kE, +1>(x, lds, tid);
+    fft<N, kE, +1>(x, lds, tid, tw);
     {
         const int t = opaque(tid);
 #pragma unroll
@@ -292,6 +288,7 @@ __global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base)
     const int gA = unpaired ? ngroups - 1 : 2 * p;
     float a[kE], b[kE];
     group_energy<N>(a, J, f, gA, s, tid);
+    const FftTwiddles<N, kE> tw = fft_twiddles<N, kE>(tid);
     {   // keep the first group's energy for the next level (same head row, same thread)
         float* dst = J.egrp + (static_cast<size_t>(f) * M + static_cast<size_t>(gA) * s) * N;
 #pragma unroll
@@ -300,7 +297,7 @@ __global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base)
     if (unpaired) return;
     group_energy<N>(b, J, f, gA + 1, s, tid);
     const size_t idx = static_cast<size_t>(f) * (M - 1) + pair_base + p;
-    (void)correlate_pair<N>(a, b, lds, redv, redi, tid, J.lags + idx, J.imax + idx);
+    (void)correlate_pair<N>(a, b, lds, redv, redi, tid, tw, J.lags + idx, J.imax + idx);
 }
 
 // Shift the rows of the second group of pair p by the pair's lag (range_alignment.cu:66-69)
@@ -328,13 +325,14 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_upd(Job J, int s, int pair_bas
 #pragma unroll
     for (int e = 0; e < kE; ++e) acc[e] = 0.0f;
     float2 x[kE];
+    const FftTwiddles<N, kE> tw = fft_twiddles<N, kE>(tid);
     // column sums of |profile| over the final rows of this chunk, accumulated in place (each
     // location is read back only by the thread that wrote it)
     float* ap = J.arp_part + (static_cast<size_t>(f) * (M / 2) + r0 / 2) * N;
     for (int m = r0; m < r1; ++m) {
         const double L = J.lagsum[static_cast<size_t>(f) * M + m] + static_cast<double>(lag);
         load_row_shifted<N>(x, J, f, m, L, tid);
-        fft<N, kE, +1>(x, lds, tid);
+        fft<N, kE, +1>(x, lds, tid, tw);
 #pragma unroll
         for (int e = 0; e < kE; ++e) acc[e] += cabs2(x[e]) * inv;
         if (final_rows) {   // S0 of row m was read before the transform's first barrier

@@ -72,6 +72,7 @@ struct Job {
     float2* czt;                  // K x M x W (chirp-z output)
     float2* colsum;               // K x W
     const float2* czt_table;      // W x F (FFT of the Bluestein chirp per range bin)
+    const float* win_range;       // N: Hamming window over range cells (genHammingVec, common.cu:260-266)
     // output
     float* img;                   // K x M x W
 };
