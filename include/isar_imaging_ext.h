@@ -55,8 +55,14 @@ typedef struct isar_options {
     int32_t quirk_q1;         /* 1 (default): slow-time window as isar_main.cu:192 applies it  */
     int32_t iterations;       /* fast-entropy iterations (reference: 120)                      */
     int32_t warm_start;       /* 1: phi0 supplied by the caller seeds the autofocus            */
-    int32_t reserved;
+    int32_t stage_mode;       /* ISAR_RUN_FULL, ISAR_RUN_FRONT or ISAR_RUN_BACK                */
 } isar_options;
+
+/* Parts of the chain a run executes. FRONT stops after the autofocus (no image); BACK repeats
+ * the autofocus from the cut and selection kept by the previous FRONT or FULL run of the same
+ * context and frame count, then forms the images (echo and velocity may be NULL; with a NULL
+ * image pointer only the autofocus runs). */
+enum isar_stage_mode { ISAR_RUN_FULL = 0, ISAR_RUN_FRONT = 1, ISAR_RUN_BACK = 2 };
 
 /* Per-frame decisions and vectors copied back by isar_get_trace. Pointers are caller-owned
  * and may be NULL. */
@@ -99,6 +105,11 @@ ISAR_API int isar_run_device(isar_context* ctx, int frames, const void* d_echo, 
 /* Host batch: copies in, runs, copies the images out; synchronous. */
 ISAR_API int isar_run_host(isar_context* ctx, int frames, const float* h_echo, const double* h_velocity,
                            const float* h_phi0, float* h_img, const isar_options* opt);
+
+/* Autofocus phases of the last run (device to device, asynchronous on `stream`): the starting
+ * phase (Doppler tracking, or the warm start) and the final phase, K*M complex float each.
+ * Either destination may be NULL. */
+ISAR_API int isar_get_phases(isar_context* ctx, int frames, void* d_phi_start, void* d_phi_final, void* stream);
 
 /* After a run has completed: decisions of frame `frame` of the last batch. */
 ISAR_API int isar_get_trace(isar_context* ctx, int frame, isar_frame_trace* out);

@@ -32,7 +32,7 @@ class _Params(C.Structure):
 
 class _Options(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("data_type", "option_alignment", "option_phase", "if_hrrp", "if_hpc",
-                                          "if_mtrc", "quirk_q1", "iterations", "warm_start", "reserved")]
+                                          "if_mtrc", "quirk_q1", "iterations", "warm_start", "stage_mode")]
 
 
 _fp = C.POINTER(C.c_float)
@@ -71,6 +71,7 @@ def lib() -> C.CDLL:
         L.isar_run_device.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.POINTER(_Options), vp]
         L.isar_run_host.argtypes = [vp, C.c_int, vp, vp, vp, vp, C.POINTER(_Options)]
         L.isar_get_trace.argtypes = [vp, C.c_int, C.POINTER(_Trace)]
+        L.isar_get_phases.argtypes = [vp, C.c_int, vp, vp, vp]
         L.isar_fft_host.argtypes = [C.c_int, C.c_int, vp, vp, C.c_int]
         L.isar_probe_enable.argtypes = [vp, C.c_int]
         L.isar_probe_read.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -115,11 +116,12 @@ class Options:
     quirk_q1: bool = True
     iterations: int = 120
     warm_start: bool = False
+    stage_mode: int = 0            # 0 full chain, 1 front (stop after the autofocus), 2 back
 
     def c(self) -> _Options:
         return _Options(self.data_type, self.option_alignment, self.option_phase, int(self.if_hrrp),
                         int(self.if_hpc), int(self.if_mtrc), int(self.quirk_q1), self.iterations,
-                        int(self.warm_start), 0)
+                        int(self.warm_start), int(self.stage_mode))
 
 
 @dataclass
@@ -205,6 +207,12 @@ class Imager:
         _check(lib().isar_run_device(self._h, frames, C.c_void_p(echo_ptr), C.c_void_p(velocity_ptr or 0),
                                      C.c_void_p(phi0_ptr or 0), C.c_void_p(img_ptr), C.byref(options.c()),
                                      C.c_void_p(stream or 0)), "isar_run_device")
+
+    def phases(self, frames: int, start_ptr: int | None, final_ptr: int | None, stream: int | None = None) -> None:
+        """Copy the autofocus start and final phases of the last run (K x M complex64 each) into
+        device buffers; asynchronous on `stream`."""
+        _check(lib().isar_get_phases(self._h, frames, C.c_void_p(start_ptr or 0), C.c_void_p(final_ptr or 0),
+                                     C.c_void_p(stream or 0)), "isar_get_phases")
 
     def probe_enable(self, stage: int) -> None:
         """Time every launch of kernel family `stage` (STAGES) with HIP events on the run stream."""

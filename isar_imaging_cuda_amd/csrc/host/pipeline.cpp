@@ -215,10 +215,17 @@ int Context::run(int frames, const void* echo, const double* velocity, const voi
                  const isar_options& opt, hipStream_t s) {
     if (frames < 1 || frames > max_frames)
         return fail(ISAR_ERR_INVALID, "frames must be in [1, " + std::to_string(max_frames) + "]");
-    if (!echo || !img) return fail(ISAR_ERR_INVALID, "null echo or image pointer");
+    const int mode = opt.stage_mode;
+    if (mode != ISAR_RUN_FULL && mode != ISAR_RUN_FRONT && mode != ISAR_RUN_BACK)
+        return fail(ISAR_ERR_INVALID, "stage_mode must be 0 (full), 1 (front) or 2 (back)");
+    if (mode != ISAR_RUN_BACK && !echo) return fail(ISAR_ERR_INVALID, "null echo pointer");
+    if (mode == ISAR_RUN_FULL && !img) return fail(ISAR_ERR_INVALID, "null image pointer");
+    if (mode == ISAR_RUN_BACK && frames != front_frames)
+        return fail(ISAR_ERR_STATE, "a back run needs a front or full run of the same frame count before it");
     if (opt.data_type != 1 && opt.data_type != 2) return fail(ISAR_ERR_INVALID, "data_type must be 1 (IFDS) or 2 (STRETCH)");
     const bool hpc = opt.if_hpc && opt.data_type == 2;
-    if (hpc && !velocity) return fail(ISAR_ERR_INVALID, "high-speed compensation needs the velocity vector");
+    if (hpc && !velocity && mode != ISAR_RUN_BACK)
+        return fail(ISAR_ERR_INVALID, "high-speed compensation needs the velocity vector");
     if (opt.warm_start && !phi0) return fail(ISAR_ERR_INVALID, "warm start needs phi0");
     if (opt.iterations < 0) return fail(ISAR_ERR_INVALID, "iterations must be >= 0");
     Job J = tmpl;
@@ -239,22 +246,26 @@ int Context::run(int frames, const void* echo, const double* velocity, const voi
     (void)hipGetLastError();
     // a family launches at most log2(M) + 1 times per run; drain the ring only when it could overflow
     if (probe_stage >= 0 && probe_used + 16 > static_cast<int>(probe_events.size()) / 2) probe_collect();
-    if (J.data_type == 1) probed(ISAR_STAGE_IFDS_PRE, s, [&] { launch_ifds_pre(J, s); });
-    probed(ISAR_STAGE_RA_PAIR0, s, [&] { launch_ra_pair0(J, s); });
-    int pair_base = M / 2;
-    for (int st = 2; st < M; st *= 2) {
-        const int ngroups = (M + st - 1) / st, npairs = ngroups / 2, unpaired = ngroups & 1;
-        probed(ISAR_STAGE_RA_CORR, s, [&] { launch_ra_corr(J, st, pair_base, npairs + unpaired, s); });
-        const int R = std::min(st, kRaChunkRows), cpp = (st + R - 1) / R;
-        if (npairs > 0) probed(ISAR_STAGE_RA_UPD, s, [&] { launch_ra_upd(J, st, pair_base, npairs * cpp, s); });
-        pair_base += npairs;
+    if (mode != ISAR_RUN_BACK) {
+        if (J.data_type == 1) probed(ISAR_STAGE_IFDS_PRE, s, [&] { launch_ifds_pre(J, s); });
+        probed(ISAR_STAGE_RA_PAIR0, s, [&] { launch_ra_pair0(J, s); });
+        int pair_base = M / 2;
+        for (int st = 2; st < M; st *= 2) {
+            const int ngroups = (M + st - 1) / st, npairs = ngroups / 2, unpaired = ngroups & 1;
+            probed(ISAR_STAGE_RA_CORR, s, [&] { launch_ra_corr(J, st, pair_base, npairs + unpaired, s); });
+            const int R = std::min(st, kRaChunkRows), cpp = (st + R - 1) / R;
+            if (npairs > 0) probed(ISAR_STAGE_RA_UPD, s, [&] { launch_ra_upd(J, st, pair_base, npairs * cpp, s); });
+            pair_base += npairs;
+        }
+        probed(ISAR_STAGE_CENTER, s, [&] { launch_center(J, s); });
+        probed(ISAR_STAGE_CUT, s, [&] { launch_cut(J, s); });
+        probed(ISAR_STAGE_PA_COLFFT, s, [&] { launch_pa_colfft(J, s); });
+        probed(ISAR_STAGE_PA_SELECT, s, [&] { launch_pa_select(J, s); });
     }
-    probed(ISAR_STAGE_CENTER, s, [&] { launch_center(J, s); });
-    probed(ISAR_STAGE_CUT, s, [&] { launch_cut(J, s); });
-    probed(ISAR_STAGE_PA_COLFFT, s, [&] { launch_pa_colfft(J, s); });
-    probed(ISAR_STAGE_PA_SELECT, s, [&] { launch_pa_select(J, s); });
     probed(ISAR_STAGE_AUTOFOCUS, s, [&] { launch_autofocus(J, s); });
-    if (J.if_mtrc) {
+    if (mode == ISAR_RUN_FRONT || (mode == ISAR_RUN_BACK && !img)) {
+        // nothing after the autofocus
+    } else if (J.if_mtrc) {
         probed(ISAR_STAGE_MTRC_ROWS, s, [&] { launch_mtrc_rows(J, s); });
         probed(ISAR_STAGE_MTRC_CZT, s, [&] { launch_mtrc_czt(J, s); });
         if (!J.quirk_q1) probed(ISAR_STAGE_COL_WINDOW, s, [&] { launch_col_window(J, s); });
@@ -265,6 +276,17 @@ int Context::run(int frames, const void* echo, const double* velocity, const voi
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
     last_frames = frames;
+    if (mode != ISAR_RUN_BACK) front_frames = frames;
+    return ISAR_OK;
+}
+
+int Context::phases(int frames, void* d_start, void* d_final, hipStream_t s) {
+    if (frames < 1 || frames > last_frames) return fail(ISAR_ERR_INVALID, "frames exceeds the last run's batch");
+    const size_t bytes = static_cast<size_t>(frames) * p.echo_num * sizeof(float2);
+    if (d_start && hipMemcpyAsync(d_start, tmpl.phi0, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return fail(ISAR_ERR_DEVICE, "phase copy failed");
+    if (d_final && hipMemcpyAsync(d_final, tmpl.phi, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return fail(ISAR_ERR_DEVICE, "phase copy failed");
     return ISAR_OK;
 }
 
@@ -326,6 +348,7 @@ ISAR_API void isar_default_options(isar_options* opt) {
     opt->quirk_q1 = 1;
     opt->iterations = 120;   // FAST_ENTROPY_ITERATION_NUM (common.cuh:73)
     opt->warm_start = 0;
+    opt->stage_mode = ISAR_RUN_FULL;
 }
 
 ISAR_API size_t isar_workspace_bytes(const isar_radar_params* p, int max_frames) {
@@ -389,6 +412,11 @@ ISAR_API int isar_run_host(isar_context* ctx, int frames, const float* h_echo, c
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("run/download: ") + hipGetErrorString(e));
     return ISAR_OK;
+}
+
+ISAR_API int isar_get_phases(isar_context* ctx, int frames, void* d_phi_start, void* d_phi_final, void* stream) {
+    if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");
+    return ctx->phases(frames, d_phi_start, d_phi_final, static_cast<hipStream_t>(stream));
 }
 
 ISAR_API int isar_get_trace(isar_context* ctx, int frame, isar_frame_trace* out) {

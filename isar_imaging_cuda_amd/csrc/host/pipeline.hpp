@@ -20,6 +20,7 @@ struct Context {
     std::vector<void*> allocations;   // everything allocated at creation
     float2* czt_table = nullptr;
     int last_frames = 0;
+    int front_frames = 0;             // batch size of the last run that formed the cut
     // kernel probe: event pairs recorded around each launch of one kernel family
     int probe_stage = -1;
     std::vector<hipEvent_t> probe_events;   // 2 per slot
@@ -48,6 +49,7 @@ struct Context {
     int run(int frames, const void* echo, const double* velocity, const void* phi0, float* img,
             const isar_options& opt, hipStream_t stream);
     int trace(int frame, isar_frame_trace* out);
+    int phases(int frames, void* d_start, void* d_final, hipStream_t s);
 };
 
 int next_pow2_ref(int n);   // common.cu:521-529

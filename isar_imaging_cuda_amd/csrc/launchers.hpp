@@ -37,6 +37,22 @@ void launch_col_window(const Job& J, hipStream_t s);
 void launch_mtrc_image(const Job& J, hipStream_t s);
 void launch_post_nomtrc(const Job& J, hipStream_t s);
 
+// learned warm start: BF16 matrix-core GEMMs with FP32 accumulation and their helpers
+void launch_gemm_f32_bf16w(int m, int n, int k, const float* a, long sam, long sak, const uint16_t* w, long sbk,
+                           long sbn, float* c, long scm, const float* bias, int relu, hipStream_t s);
+void launch_gemm_f32_bf16w_masked(int m, int n, int k, const float* a, long sam, long sak, const uint16_t* w,
+                                  long sbk, long sbn, float* c, long scm, const float* mask, long smm, hipStream_t s);
+void launch_gemm_f32_f32(int m, int n, int k, const float* a, long sam, long sak, const float* b, long sbk, long sbn,
+                         float* c, long scm, hipStream_t s);
+void launch_colsum(int rows, int cols, const float* dz, float* db, hipStream_t s);
+void launch_adam(int n, float* p, const float* g, float* m1, float* m2, uint16_t* pb, const int* bf_index, float lr,
+                 float beta1, float beta2, float eps, float c1, float c2, hipStream_t s);
+void launch_to_bf16(int n, const float* p, uint16_t* pb, const int* bf_index, hipStream_t s);
+void launch_mse(int rows, int cols, const float* pred, const float* target, float* grad, double* loss, hipStream_t s);
+void launch_phase_change(int frames, int M, const float2* a, const float2* b, float* out, int stride, hipStream_t s);
+void launch_compose_phase(int frames, int M, const float2* phi_b, const float* pred, float2* phi_w, hipStream_t s);
+void launch_image_entropy(int frames, int n, const float* img, double* out, hipStream_t s);
+
 // standalone batched FFT used by the tests: n-point transforms of `batch` contiguous rows
 bool debug_fft(int n, int dir, const float2* in, float2* out, int batch, hipStream_t s);
 

@@ -25,7 +25,7 @@ def _declared(header: str) -> list[str]:
     return sorted(set(names))
 
 
-@pytest.mark.parametrize("header", ["isar_imaging_export.h", "isar_imaging_ext.h", "isar_synth.h"])
+@pytest.mark.parametrize("header", ["isar_imaging_export.h", "isar_imaging_ext.h", "isar_learned.h", "isar_synth.h"])
 def test_library_exports_every_declared_entry_point(header):
     names = _declared(header)
     assert names, header

@@ -1,0 +1,246 @@
+"""Learned warm start: the FP64 network oracle pinned by finite differences and closed forms (CPU),
+and the matrix-core network kernels, phase helpers and staged pipeline checked against it and
+against the plain chain (MI355X)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as MO
+
+
+# ------------------------------------------------------------------------------------------
+# CPU: pin the oracle
+# ------------------------------------------------------------------------------------------
+def test_oracle_gradients_match_finite_differences():
+    widths = [6, 5, 4, 3]
+    rng = np.random.default_rng(3)
+    params = rng.standard_normal(sum(i * o + o for i, o in zip(widths[:-1], widths[1:])))
+    x = rng.standard_normal((7, widths[0]))
+    y = rng.standard_normal((7, widths[-1]))
+
+    def loss(p):
+        out, _ = MO.forward(p, widths, x, bf16=False)
+        return MO.mse(out, y)[0]
+
+    out, acts = MO.forward(params, widths, x, bf16=False)
+    _, g_out = MO.mse(out, y)
+    g = MO.backward(params, widths, acts, g_out, bf16=False)
+    h = 1e-6
+    fd = np.array([(loss(params + h * e) - loss(params - h * e)) / (2 * h) for e in np.eye(params.size)])
+    assert np.allclose(g, fd, rtol=1e-6, atol=1e-8)
+
+
+def test_oracle_bf16_rounding_is_nearest_even():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -7, -2.5, 3.0e-3], np.float32)
+    r = MO.bf16_round(v)
+    assert r[0] == 1.0
+    assert r[1] == 1.0                   # halfway, mantissa ...0 stays
+    assert r[2] == 1.0 + 2 ** -6         # halfway, odd mantissa rounds up to even
+    assert r[3] == 1.0 + 2 ** -7         # representable
+    assert r[4] == -2.5
+    assert abs(r[5] - 3.0e-3) <= 3.0e-3 * 2 ** -8
+
+
+def test_oracle_first_adam_step_is_signed_lr():
+    rng = np.random.default_rng(1)
+    p = rng.standard_normal(50)
+    g = rng.standard_normal(50)
+    p1, _, _ = MO.adam(p, g, np.zeros(50), np.zeros(50), 1, lr=0.01, eps=0.0)
+    assert np.allclose(p1, p - 0.01 * np.sign(g))
+
+
+def test_oracle_relu_network_is_piecewise_linear():
+    widths = [4, 8, 2]
+    p = MO.init_params(widths, 5).astype(np.float64)
+    x = np.random.default_rng(2).standard_normal((3, 4))
+    a, _ = MO.forward(p, widths, x, bf16=False)
+    b, _ = MO.forward(p, widths, 2 * x, bf16=False)
+    # zero biases: a ReLU network without biases is positively homogeneous
+    assert np.allclose(b, 2 * a)
+
+
+# ------------------------------------------------------------------------------------------
+# GPU
+# ------------------------------------------------------------------------------------------
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _net_roundtrip(widths, batch, seed):
+    torch = _torch()
+    from isar_imaging_cuda_amd import learned as L
+    rng = np.random.default_rng(seed)
+    params = MO.init_params(widths, seed) + 0.05 * rng.standard_normal(
+        sum(i * o + o for i, o in zip(widths[:-1], widths[1:]))).astype(np.float32)
+    x = rng.standard_normal((batch, widths[0])).astype(np.float32)
+    y = rng.standard_normal((batch, widths[-1])).astype(np.float32)
+    net = L.Mlp(widths, batch)
+    net.set_params(params)
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(x).to(dev)
+    dy = torch.from_numpy(y).to(dev)
+    out = torch.zeros((batch, widths[-1]), dtype=torch.float32, device=dev)
+    grad = torch.zeros_like(out)
+    loss = torch.zeros(batch, dtype=torch.float64, device=dev)
+    s = torch.cuda.current_stream().cuda_stream
+    net.forward(batch, dx.data_ptr(), out.data_ptr(), s)
+    L.mse(batch, widths[-1], out.data_ptr(), dy.data_ptr(), grad.data_ptr(), loss.data_ptr(), s)
+    net.backward(batch, grad.data_ptr(), s)
+    torch.cuda.synchronize()
+    return net, params, x, y, out.cpu().numpy(), grad.cpu().numpy(), float(loss.sum().item())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("widths,batch", [([16, 16], 16), ([40, 24], 37), ([64, 32, 32, 48], 40),
+                                          ([512, 128, 128, 128, 128, 512], 64)])
+def test_network_forward_and_gradients_match_oracle(widths, batch):
+    print(f"[case] network {widths} batch {batch}", flush=True)
+    net, params, x, y, out, grad_out, loss = _net_roundtrip(widths, batch, len(widths) * 7 + batch)
+    # One layer: identical BF16 operands on both sides, so only the FP32 accumulation differs.
+    # Deeper: the device rounds its FP32 activations to BF16 where the oracle rounds FP64 ones, so
+    # an activation next to a rounding boundary can land one BF16 step away (2^-8 relative).
+    tol_out, tol_grad = (1e-5, 1e-5) if len(widths) == 2 else (3e-3, 5e-3)
+    ref, _ = MO.forward(params, widths, x, bf16=True)
+    assert np.abs(out - ref).max() <= tol_out * max(1.0, np.abs(ref).max())
+    ref_loss, ref_gout = MO.mse(out, y)
+    assert np.allclose(grad_out, ref_gout, rtol=1e-5, atol=1e-9)
+    assert abs(loss - ref_loss) <= 1e-6 * ref_loss
+    g = net.get_grads()
+    _, acts = MO.forward(params, widths, x, bf16=True)
+    ref_g = MO.backward(params, widths, acts, grad_out.astype(np.float64), bf16=True)
+    scale = np.abs(ref_g).max()
+    assert np.abs(g - ref_g).max() <= tol_grad * scale, f"max grad diff {np.abs(g - ref_g).max():.3e} scale {scale:.3e}"
+
+
+@pytest.mark.gpu
+def test_adam_step_matches_oracle():
+    torch = _torch()
+    widths = [32, 16, 8]
+    net, params, x, y, out, grad_out, _ = _net_roundtrip(widths, 24, 11)
+    g = net.get_grads().astype(np.float64)
+    s = torch.cuda.current_stream().cuda_stream
+    net.adam_step(1e-3, 1, stream=s)
+    net.adam_step(1e-3, 2, stream=s)   # same gradient twice
+    torch.cuda.synchronize()
+    p_ref, m1, m2 = MO.adam(params.astype(np.float64), g, np.zeros_like(g), np.zeros_like(g), 1)
+    p_ref, _, _ = MO.adam(p_ref, g, m1, m2, 2)
+    assert np.abs(net.get_params() - p_ref).max() <= 1e-6
+
+
+@pytest.mark.gpu
+def test_phase_helpers_match_numpy():
+    torch = _torch()
+    from isar_imaging_cuda_amd import learned as L
+    rng = np.random.default_rng(4)
+    K, M = 5, 96
+    a = np.exp(1j * rng.uniform(-np.pi, np.pi, (K, M))).astype(np.complex64)
+    b = np.exp(1j * rng.uniform(-np.pi, np.pi, (K, M))).astype(np.complex64)
+    pred = rng.standard_normal((K, 2 * M)).astype(np.float32)
+    pred[0, :2] = 0.0
+    dev = torch.device("cuda:0")
+    da = torch.from_numpy(a.view(np.float32)).to(dev)
+    db = torch.from_numpy(b.view(np.float32)).to(dev)
+    dp = torch.from_numpy(pred).to(dev)
+    out = torch.zeros((K, 2 * M), dtype=torch.float32, device=dev)
+    w = torch.zeros((K, 2 * M), dtype=torch.float32, device=dev)
+    s = torch.cuda.current_stream().cuda_stream
+    L.phase_change(K, M, da.data_ptr(), db.data_ptr(), out.data_ptr(), s)
+    L.compose_phase(K, M, db.data_ptr(), dp.data_ptr(), w.data_ptr(), s)
+    torch.cuda.synchronize()
+    ch = out.cpu().numpy().view(np.complex64)
+    assert np.abs(ch - b * np.conj(a)).max() < 1e-6
+    q = pred.view(np.complex64)
+    ref = np.where(np.abs(q) > 0, b * q / np.where(np.abs(q) > 0, np.abs(q), 1), b)
+    assert np.abs(w.cpu().numpy().view(np.complex64) - ref).max() < 2e-6
+
+
+@pytest.mark.gpu
+def test_image_entropy_matches_oracle():
+    torch = _torch()
+    from isar_imaging_cuda_amd import learned as L
+    from oracle import oracle as O
+    rng = np.random.default_rng(8)
+    imgs = np.abs(rng.standard_normal((3, 64, 512))).astype(np.float32)
+    imgs[2] = 0.0
+    imgs[2, 5, 7] = 3.0                      # a single bright cell: entropy 0
+    dev = torch.device("cuda:0")
+    d = torch.from_numpy(imgs).to(dev)
+    out = torch.zeros(3, dtype=torch.float64, device=dev)
+    L.image_entropy(3, 64 * 512, d.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    e = out.cpu().numpy()
+    for k in range(2):
+        assert abs(e[k] - O.image_entropy(imgs[k])) <= 1e-9 * abs(e[k])
+    assert e[2] == 0.0
+
+
+def _c4_batch(K, seed0):
+    from isar_imaging_cuda_amd import learned as L
+    echo, vel, h = L.synth_frames(4, list(range(seed0, seed0 + K)))
+    return echo, vel, h
+
+
+@pytest.mark.gpu
+def test_staged_run_matches_full_chain():
+    """The front's iterations, taken one at a time, equal the full chain run with that many
+    iterations; front + back (plain restart, 120 iterations) equals the full chain."""
+    torch = _torch()
+    import isar_imaging_cuda_amd as isar
+    from isar_imaging_cuda_amd import learned as L
+    K = 8
+    echo, vel, h = _c4_batch(K, 500)
+    M, N = echo.shape[1:]
+    print(f"[case] staged vs full, {K} frames {M}x{N}", flush=True)
+    params = isar.RadarParams(M, N, h["band_width"], h["fc"], h["Tp"], h["Fs"])
+    dev = torch.device("cuda:0")
+    e = torch.from_numpy(echo.view(np.float32)).to(dev)
+    v = torch.from_numpy(vel).to(dev)
+    s = torch.cuda.current_stream().cuda_stream
+    ws = L.WarmStartImager(params, K, front_iters=2)
+    img_full = torch.zeros((K, M, 512), dtype=torch.float32, device=dev)
+    img_back = torch.zeros_like(img_full)
+    ph_full = [torch.zeros((K, M, 2), dtype=torch.float32, device=dev) for _ in range(3)]
+    for it in (1, 2):
+        ws.imager.run_device(e.data_ptr(), v.data_ptr(), img_full.data_ptr(), K, isar.Options(iterations=it), stream=s)
+        ws.imager.phases(K, ph_full[0].data_ptr(), ph_full[it].data_ptr(), s)
+    ws.front(K, e.data_ptr(), v.data_ptr(), s)
+    torch.cuda.synchronize()
+    for i in range(3):
+        assert torch.equal(ws.phis[i], ph_full[i]), f"phase after {i} iterations"
+    ws.imager.run_device(e.data_ptr(), v.data_ptr(), img_full.data_ptr(), K, isar.Options(), stream=s)
+    ws.front(K, e.data_ptr(), v.data_ptr(), s)
+    ws.back(K, img_back.data_ptr(), 120, s, warm=False)
+    torch.cuda.synchronize()
+    assert torch.equal(img_full, img_back)
+
+
+@pytest.mark.gpu
+def test_training_on_pipeline_frames_reduces_loss():
+    _torch()
+    from isar_imaging_cuda_amd import learned as L
+    print("[case] train 64 frames, 6 epochs", flush=True)
+    feats, targets = L.make_dataset(list(range(900, 964)), chunk=64, log=lambda m: None)
+    assert feats.shape == (64, 512 * L.FRONT_ITERS) and targets.shape == (64, 512)
+    _, hist = L.train(feats, targets, epochs=6, batch=32, lr=1e-3, log=lambda m: None)
+    assert hist[-1] < hist[0]
+
+
+@pytest.mark.gpu
+def test_learned_warm_start_needs_fewer_iterations():
+    """The shipped network against the plain iteration on held-out frames: the learned chain
+    reaches the plain 120-iteration image entropy (within 1e-5 relative) in fewer iterations."""
+    _torch()
+    from isar_imaging_cuda_amd import learned as L
+    try:
+        params = L.load_weights()
+    except Exception as e:
+        pytest.skip(str(e))
+    print("[case] learned vs plain, 64 held-out frames", flush=True)
+    res = L.evaluate(params, list(range(20_000_001, 20_000_065)), chunk=64, log=lambda m: None)
+    print(res)
+    assert res["accepted_fraction"] >= 0.9
+    assert res["learned_mean_iterations"] < res["plain_mean_iterations"]
