@@ -102,6 +102,19 @@ ISAR_API size_t isar_workspace_bytes(const isar_radar_params* p, int max_frames)
 ISAR_API int isar_run_device(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity,
                              const void* d_phi0, float* d_img, const isar_options* opt, void* stream);
 
+/* Device-resident batch whose frames are windows of a longer recording: frame f uses echo rows
+ * (and velocities) d_frame_rows[f] .. d_frame_rows[f] + M - 1 of d_echo (N complex floats per
+ * row) and writes its M x W image at d_img + d_frame_img[f] floats. Both tables are device
+ * arrays of `frames` int64 entries; either may be NULL for contiguous frames. */
+ISAR_API int isar_run_frames(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity,
+                             const int64_t* d_frame_rows, const void* d_phi0, float* d_img,
+                             const int64_t* d_frame_img, const isar_options* opt, void* stream);
+
+/* Rows of the range-alignment update handled per workgroup (2, 4, 8 or 16; default 8). The
+ * choice changes the summation order of the group energies, not the algorithm. */
+ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows);
+ISAR_API int isar_get_ra_chunk(isar_context* ctx);
+
 /* Host batch: copies in, runs, copies the images out; synchronous. */
 ISAR_API int isar_run_host(isar_context* ctx, int frames, const float* h_echo, const double* h_velocity,
                            const float* h_phi0, float* h_img, const isar_options* opt);

@@ -213,6 +213,12 @@ int Context::init(const isar_radar_params& params, int frames) {
 
 int Context::run(int frames, const void* echo, const double* velocity, const void* phi0, float* img,
                  const isar_options& opt, hipStream_t s) {
+    return run_frames(frames, echo, velocity, nullptr, phi0, img, nullptr, opt, s);
+}
+
+int Context::run_frames(int frames, const void* echo, const double* velocity, const long long* frame_row,
+                        const void* phi0, float* img, const long long* frame_img, const isar_options& opt,
+                        hipStream_t s) {
     if (frames < 1 || frames > max_frames)
         return fail(ISAR_ERR_INVALID, "frames must be in [1, " + std::to_string(max_frames) + "]");
     const int mode = opt.stage_mode;
@@ -239,6 +245,9 @@ int Context::run(int frames, const void* echo, const double* velocity, const voi
     J.phi0_given = opt.warm_start ? 1 : 0;
     J.echo = static_cast<const float2*>(echo);
     J.velocity = velocity;
+    J.frame_row = frame_row;
+    J.frame_img = frame_img;
+    J.ra_chunk = ra_chunk;
     J.phi0_in = static_cast<const float2*>(phi0);
     J.img = img;
     const int M = J.M;
@@ -253,7 +262,7 @@ int Context::run(int frames, const void* echo, const double* velocity, const voi
         for (int st = 2; st < M; st *= 2) {
             const int ngroups = (M + st - 1) / st, npairs = ngroups / 2, unpaired = ngroups & 1;
             probed(ISAR_STAGE_RA_CORR, s, [&] { launch_ra_corr(J, st, pair_base, npairs + unpaired, s); });
-            const int R = std::min(st, kRaChunkRows), cpp = (st + R - 1) / R;
+            const int R = std::min(st, J.ra_chunk), cpp = (st + R - 1) / R;
             if (npairs > 0) probed(ISAR_STAGE_RA_UPD, s, [&] { launch_ra_upd(J, st, pair_base, npairs * cpp, s); });
             pair_base += npairs;
         }
@@ -413,6 +422,26 @@ ISAR_API int isar_run_host(isar_context* ctx, int frames, const float* h_echo, c
     if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("run/download: ") + hipGetErrorString(e));
     return ISAR_OK;
 }
+
+ISAR_API int isar_run_frames(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity,
+                             const int64_t* d_frame_rows, const void* d_phi0, float* d_img,
+                             const int64_t* d_frame_img, const isar_options* opt, void* stream) {
+    if (!ctx || !opt) return isar::fail(ISAR_ERR_INVALID, "null context or options");
+    static_assert(sizeof(long long) == sizeof(int64_t), "frame tables are int64");
+    return ctx->run_frames(frames, d_echo, d_velocity, reinterpret_cast<const long long*>(d_frame_rows), d_phi0,
+                           d_img, reinterpret_cast<const long long*>(d_frame_img), *opt,
+                           static_cast<hipStream_t>(stream));
+}
+
+ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows) {
+    if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");
+    if (rows != 2 && rows != 4 && rows != 8 && rows != 16)
+        return isar::fail(ISAR_ERR_INVALID, "ra chunk must be 2, 4, 8 or 16 rows");
+    ctx->ra_chunk = rows;
+    return ISAR_OK;
+}
+
+ISAR_API int isar_get_ra_chunk(isar_context* ctx) { return ctx ? ctx->ra_chunk : 0; }
 
 ISAR_API int isar_get_phases(isar_context* ctx, int frames, void* d_phi_start, void* d_phi_final, void* stream) {
     if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");

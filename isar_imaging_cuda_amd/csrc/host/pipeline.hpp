@@ -48,6 +48,12 @@ struct Context {
     int init(const isar_radar_params& params, int frames);
     int run(int frames, const void* echo, const double* velocity, const void* phi0, float* img,
             const isar_options& opt, hipStream_t stream);
+    // frame f reads echo rows frame_row[f] .. + M - 1 and writes its image at img + frame_img[f]
+    // (either table may be null: contiguous frames); both tables are device arrays of K entries
+    int run_frames(int frames, const void* echo, const double* velocity, const long long* frame_row,
+                   const void* phi0, float* img, const long long* frame_img, const isar_options& opt,
+                   hipStream_t stream);
+    int ra_chunk = kRaChunkRowsDefault;   // tunable: rows per workgroup in the alignment update
     int trace(int frame, isar_frame_trace* out);
     int phases(int frames, void* d_start, void* d_final, hipStream_t s);
 };

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kCenterThreads) void k_center(Job J) {
     float* redv = srt + N;
     int* redi = reinterpret_cast<int*>(redv + 32);
     // column sums of |S| and max |S| arrive as partials per chunk of rows that finished moving:
-    // rows 0-1 (head 0) and, at every merge stride s >= 2, chunks of min(s, kRaChunkRows) rows
+    // rows 0-1 (head 0) and, at every merge stride s >= 2, chunks of min(s, ra_chunk) rows
     // starting at s (the second group of pair 0). Partials are stored at head row / 2.
     const float* ap = J.arp_part + static_cast<size_t>(f) * (M / 2) * N;
     const float* mp = J.max_part + static_cast<size_t>(f) * (M / 2);
@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kCenterThreads) void k_center(Job J) {
     if (tid == 0) {
         gm = mp[0];
         for (int s = 2; s < M; s *= 2)
-            for (int h = s; h < min(2 * s, M); h += min(s, kRaChunkRows)) gm = fmaxf(gm, mp[h / 2]);
+            for (int h = s; h < min(2 * s, M); h += min(s, J.ra_chunk)) gm = fmaxf(gm, mp[h / 2]);
     }
     gm = block_max<NT>(gm, redv);
     const float inv = gm > 0.0f ? 1.0f / gm : 0.0f;
@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kCenterThreads) void k_center(Job J) {
     for (int n = tid; n < N; n += NT) {
         float acc = ap[n];
         for (int s = 2; s < M; s *= 2)
-            for (int h = s; h < min(2 * s, M); h += min(s, kRaChunkRows)) acc += ap[static_cast<size_t>(h / 2) * N + n];
+            for (int h = s; h < min(2 * s, M); h += min(s, J.ra_chunk)) acc += ap[static_cast<size_t>(h / 2) * N + n];
         const float v = acc * inv * invM;
         arp[n] = v;
         srt[n] = v;

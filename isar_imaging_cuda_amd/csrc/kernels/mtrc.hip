@@ -227,7 +227,7 @@ __global__ __launch_bounds__(kRowsPerBlock * W / kRowE) void k_mtrc_image(Job J)
     for (int e = 0; e < kRowE; ++e) x[e] = cr[row_in<W>(lane, e)];
     fft<W, kRowE, -1>(x, buf, lane);
     if (valid) {
-        float* dst = J.img + (static_cast<size_t>(f) * M + k) * W;
+        float* dst = image_of(J, f) + static_cast<size_t>(k) * W;
 #pragma unroll
         for (int e = 0; e < kRowE; ++e) {
             int j = (row_out<W>(lane, e) + W / 2) & (W - 1);
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(256) void k_post_nomtrc(Job J) {
         fo[r * (CT + 1) + g] = cabsf_(x[e]);
     }
     __syncthreads();
-    float* D = J.img + static_cast<size_t>(f) * M * W;
+    float* D = image_of(J, f);
     for (int idx = tid; idx < M * CT; idx += NT) {
         const int m = idx / CT, c = idx % CT;
         D[static_cast<size_t>(m) * W + n0 + c] = fo[m * (CT + 1) + c];

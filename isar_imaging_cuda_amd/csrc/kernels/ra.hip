@@ -29,10 +29,11 @@ __device__ __forceinline__ int out_pos(int tid, int e) {
 template <int N>
 __device__ __forceinline__ void load_row_raw(float2 (&x)[kE], const Job& J, int f, int m, int tid_) {
     const int tid = opaque(tid_);
-    const size_t off = (static_cast<size_t>(f) * J.M + m) * N;
-    const float2* row = (J.data_type == 1 ? J.pre : J.echo) + off;
+    const long long erow = echo_first_row(J, f) + m;
+    const float2* row = J.data_type == 1 ? J.pre + (static_cast<size_t>(f) * J.M + m) * N
+                                         : J.echo + static_cast<size_t>(erow) * N;
     const bool hpc = J.if_hpc && J.data_type == 2;
-    const double vk = hpc ? J.hpc_k * J.velocity[static_cast<size_t>(f) * J.M + m] : 0.0;
+    const double vk = hpc ? J.hpc_k * J.velocity[erow] : 0.0;
     const bool hrrp_window = J.if_hrrp && J.data_type == 2;
 #pragma unroll
     for (int e = 0; e < kE; ++e) {
@@ -91,7 +92,7 @@ __device__ __forceinline__ void group_energy(float (&a)[kE], const Job& J, int f
         const int r0 = (2 * g + 1) * s2;
         if (r0 < M) {
             const int r1 = min(r0 + s2, M);
-            const int R = min(s2, kRaChunkRows);
+            const int R = min(s2, J.ra_chunk);
             for (int h = r0; h < r1; h += R) {
                 const float* pp = J.epart + (static_cast<size_t>(f) * M + h) * N;
 #pragma unroll
@@ -165,9 +166,10 @@ __global__ __launch_bounds__(N / kE) void k_ifds_pre(Job J) {
     HIP_DYNAMIC_SHARED(float2, lds);
     const int m = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
     const size_t off = (static_cast<size_t>(f) * J.M + m) * N;
+    const float2* src = J.echo + static_cast<size_t>(echo_first_row(J, f) + m) * N;
     float2 x[kE];
 #pragma unroll
-    for (int e = 0; e < kE; ++e) x[e] = J.echo[off + ((in_pos<N>(tid, e) + N / 2) & (N - 1))];
+    for (int e = 0; e < kE; ++e) x[e] = src[(in_pos<N>(tid, e) + N / 2) & (N - 1)];
     fft<N, kE, +1>(x, lds, tid);
     const float inv = 1.0f / N;
 #pragma unroll
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base)
 }
 
 // Shift the rows of the second group of pair p by the pair's lag (range_alignment.cu:66-69)
-// and accumulate their new energies; each workgroup handles up to kRaChunkRows rows. The rows of
+// and accumulate their new energies; each workgroup handles up to J.ra_chunk rows. The rows of
 // pair 0 move for the last time at this level: their aligned profiles replace their spectra and
 // their column sums feed HRRP centring.
 template <int N>
@@ -309,7 +311,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_upd(Job J, int s, int pair_bas
     HIP_DYNAMIC_SHARED(float2, lds);
     const int b = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
     const int M = J.M;
-    const int R = min(s, kRaChunkRows);
+    const int R = min(s, J.ra_chunk);
     const int cpp = (s + R - 1) / R;
     const int p = b / cpp, i = b % cpp;
     const int gend = min((2 * p + 2) * s, M);

@@ -14,7 +14,7 @@
 namespace isar {
 
 constexpr int kRangeCut = 512;        // RANGE_NUM_CUT (common.cuh:71)
-constexpr int kRaChunkRows = 8;       // rows per workgroup in the alignment update kernel
+constexpr int kRaChunkRowsDefault = 8;   // rows per workgroup in the alignment update kernel
 constexpr int kCutBandRows = 16;      // rows per workgroup when cutting the range window
 
 // Per-frame scalar decisions written by the device (one record per frame).
@@ -41,11 +41,14 @@ struct Job {
     int data_type;                // 1 IFDS, 2 STRETCH
     int if_hpc, if_hrrp, if_mtrc, quirk_q1, iterations;
     int phi0_given;
+    int ra_chunk;                 // rows per workgroup in the alignment update (2, 4, 8 or 16)
     double hpc_k;                 // HPC phase in half-turns = hpc_k * v * n^2
     double czt_c0, czt_posa;      // alpha_n = 1 - c0 + posa * n
     // inputs
-    const float2* echo;           // K x M x N
-    const double* velocity;       // K x M
+    const float2* echo;           // frame f: rows frame_row[f] .. + M - 1 (f M when frame_row is null), N each
+    const double* velocity;       // indexed like the echo rows
+    const long long* frame_row;   // K first echo rows, or null
+    const long long* frame_img;   // K image offsets in floats, or null (f M W)
     const float2* phi0_in;        // K x M (optional warm start)
     // workspaces
     float2* pre;                  // K x M x N: IFDS pre-stage output, then the row spectra S0
@@ -76,5 +79,13 @@ struct Job {
     // output
     float* img;                   // K x M x W
 };
+
+// First echo row of frame f and the start of its image.
+__host__ __device__ inline long long echo_first_row(const Job& J, int f) {
+    return J.frame_row ? J.frame_row[f] : static_cast<long long>(f) * J.M;
+}
+__host__ __device__ inline float* image_of(const Job& J, int f) {
+    return J.img + (J.frame_img ? J.frame_img[f] : static_cast<long long>(f) * J.M * J.W);
+}
 
 }  // namespace isar

@@ -228,7 +228,7 @@ def _radar(h: dict, M: int, N: int) -> RadarParams:
     return RadarParams(M, N, h["band_width"], h["fc"], h["Tp"], h["Fs"])
 
 
-def make_dataset(seeds: list[int], chunk: int = 256, config: int = 4, log=print):
+def make_dataset(seeds: list[int], chunk: int = 256, config: int = 4, log=print, front_iters: int = FRONT_ITERS):
     """Features (the phase changes of the first FRONT_ITERS plain iterations) and targets
     (phi120 conj(phi_F), the remaining change of the plain iteration) for the frames of `seeds`
     (synthesised here, imaged on the GPU)."""
@@ -242,7 +242,7 @@ def make_dataset(seeds: list[int], chunk: int = 256, config: int = 4, log=print)
         echo, vel, h = synth_frames(config, part)
         K, M, N = echo.shape
         if ws is None:
-            ws = WarmStartImager(_radar(h, M, N), chunk)
+            ws = WarmStartImager(_radar(h, M, N), chunk, front_iters=front_iters)
             phi_end = torch.zeros((chunk, M, 2), dtype=torch.float32, device=dev)
         e = torch.from_numpy(echo.view(np.float32)).to(dev)
         v = torch.from_numpy(vel).to(dev)
@@ -302,7 +302,7 @@ def iterations_to_reach(ent: np.ndarray, target: np.ndarray) -> np.ndarray:
 
 
 def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int = 4, max_iter: int = 120,
-             log=print) -> dict:
+             log=print, front_iters: int = FRONT_ITERS) -> dict:
     """Iterations until the image entropy is within 1e-5 relative of the plain 120-iteration
     result, plain (from phi0) versus learned (FRONT_ITERS iterations, the network, then the
     iteration from the prediction), on the frames of `seeds`. A learned frame that never reaches the plain
@@ -316,7 +316,7 @@ def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int
         echo, vel, h = synth_frames(config, part)
         K, M, N = echo.shape
         if ws is None:
-            ws = WarmStartImager(_radar(h, M, N), chunk, params)
+            ws = WarmStartImager(_radar(h, M, N), chunk, params, front_iters=front_iters)
             img = torch.zeros((chunk, M, ws.W), dtype=torch.float32, device=dev)
             ent = torch.zeros(chunk, dtype=torch.float64, device=dev)
         e = torch.from_numpy(echo.view(np.float32)).to(dev)
@@ -356,12 +356,13 @@ def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int
                                                  / np.concatenate(e_plain_all)))}
 
 
-def load_weights(path: Path | str | None = None, M: int = 256) -> np.ndarray:
+def load_weights(path: Path | str | None = None, M: int = 256) -> tuple[np.ndarray, int]:
+    """Trained parameters and the number of front iterations they were trained for."""
     p = Path(path) if path else WEIGHTS_DIR / f"warm_start_m{M}.npz"
     if not p.exists():
         raise ImagingError(f"{p} is missing: train the warm-start network first")
     with np.load(p, allow_pickle=False) as z:
-        return np.asarray(z["params"], dtype=np.float32)
+        return np.asarray(z["params"], dtype=np.float32), int(z["front_iters"])
 
 
 def main(argv: list[str] | None = None) -> int:
@@ -373,6 +374,7 @@ def main(argv: list[str] | None = None) -> int:
     t.add_argument("--seed0", type=int, default=1)
     t.add_argument("--out", default=str(WEIGHTS_DIR / "warm_start_m256.npz"))
     t.add_argument("--eval-frames", type=int, default=256)
+    t.add_argument("--front-iters", type=int, default=FRONT_ITERS)
     ev = sub.add_parser("eval")
     ev.add_argument("--weights", default=None)
     ev.add_argument("--frames", type=int, default=1000)
@@ -381,18 +383,20 @@ def main(argv: list[str] | None = None) -> int:
     log = lambda m: print(m, flush=True)
     if a.cmd == "train":
         t0 = time.time()
-        feats, targets = make_dataset(list(range(a.seed0, a.seed0 + a.frames)), log=log)
+        feats, targets = make_dataset(list(range(a.seed0, a.seed0 + a.frames)), log=log, front_iters=a.front_iters)
         params, hist = train(feats, targets, epochs=a.epochs, log=log)
         out = Path(a.out)
         out.parent.mkdir(parents=True, exist_ok=True)
         np.savez(out, params=params, loss=np.asarray(hist), frames=a.frames, epochs=a.epochs,
-                 front_iters=FRONT_ITERS)
+                 front_iters=a.front_iters)
         log(f"[train] saved {out} after {time.time() - t0:.1f} s")
         if a.eval_frames:
-            res = evaluate(params, list(range(10_000_001, 10_000_001 + a.eval_frames)), log=log)
+            res = evaluate(params, list(range(10_000_001, 10_000_001 + a.eval_frames)), log=log,
+                           front_iters=a.front_iters)
             log(json.dumps(res))
     else:
-        res = evaluate(load_weights(a.weights), list(range(a.seed0, a.seed0 + a.frames)), log=log)
+        params, fi = load_weights(a.weights)
+        res = evaluate(params, list(range(a.seed0, a.seed0 + a.frames)), log=log, front_iters=fi)
         log(json.dumps(res))
     return 0
 

@@ -236,11 +236,12 @@ def test_learned_warm_start_needs_fewer_iterations():
     _torch()
     from isar_imaging_cuda_amd import learned as L
     try:
-        params = L.load_weights()
+        params, front_iters = L.load_weights()
     except Exception as e:
         pytest.skip(str(e))
     print("[case] learned vs plain, 64 held-out frames", flush=True)
-    res = L.evaluate(params, list(range(20_000_001, 20_000_065)), chunk=64, log=lambda m: None)
+    res = L.evaluate(params, list(range(20_000_001, 20_000_065)), chunk=64, log=lambda m: None,
+                     front_iters=front_iters)
     print(res)
     assert res["accepted_fraction"] >= 0.9
     assert res["learned_mean_iterations"] < res["plain_mean_iterations"]
