@@ -110,6 +110,56 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
+def learned_vs_plain(frames: int = 64) -> dict:
+    """Autofocus iterations to reach the plain 120-iteration image entropy (within 1e-5 relative)
+    on held-out 256 x 2048 frames (range-aligned, per-pulse phase errors uniform in [-pi, pi],
+    15 dB SNR): plain from the Doppler phase versus the shipped warm-start network."""
+    from isar_imaging_cuda_amd import learned as L
+    params, front_iters = L.load_weights()
+    res = L.evaluate(params, list(range(30_000_001, 30_000_001 + frames)), chunk=frames, log=lambda m: None,
+                     front_iters=front_iters)
+    res["front_iterations"] = front_iters
+    res["workload"] = "256 x 2048 complex64 frames, held-out seeds"
+    return res
+
+
+def subapertures(count: int = 64, replays: int = 10) -> dict:
+    """64 sliding windows of {128, 192, 256, 384, 512} pulses of one 8192 x 4096 complex64 dwell
+    imaged by one sub-aperture plan (graph replay after autotuning)."""
+    import torch
+
+    import isar_imaging_cuda_amd as isar
+    from isar_imaging_cuda_amd import subaperture as SA
+    from isar_imaging_cuda_amd import synth
+
+    cfg = synth.preset(3)
+    echo, vel = synth.frame(cfg)
+    h = synth.radar_header(cfg)
+    P, N = echo.shape
+    first, pulses = SA.sliding_windows(P, count, seed=7)
+    plan = SA.SubaperturePlan(isar.RadarParams(256, N, h["band_width"], h["fc"], h["Tp"], h["Fs"]), P, first, pulses)
+    dev = torch.device("cuda:0")
+    d_echo = torch.from_numpy(echo.view(np.float32)).to(dev)
+    d_vel = torch.from_numpy(vel).to(dev)
+    img = torch.zeros(plan.image_floats, dtype=torch.float32, device=dev)
+    chosen = plan.autotune(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), repeats=2)
+    s = torch.cuda.current_stream(dev)
+    plan.run(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), graph=True, stream=s.cuda_stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(replays):
+        plan.run(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), graph=True, stream=s.cuda_stream)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / replays
+    plan.close()
+    return {"workload": f"{count} windows of one {P} x {N} dwell, pulse counts {sorted(set(pulses.tolist()))}",
+            "ms_per_graph_launch": round(ms, 4), "windows_per_s": round(count / (ms / 1e3), 1),
+            "pulses_per_s": round(float(pulses.sum()) / (ms / 1e3), 1),
+            "ra_chunk_rows": {int(m): int(c) for m, c in zip(plan.group_pulses, chosen)}}
+
+
 def load_traffic(kernel: str, frames: int) -> dict | None:
     """HBM traffic per launch of `kernel` at this workload from a committed PMC summary
     (profiles/traffic_<kernel>_256x4096.json, written from two rocprofv3 --pmc runs and scaled by
@@ -133,6 +183,11 @@ def main() -> int:
     ap.add_argument("--probe-steps", type=int, default=3, help="extra steps timed per launch for the roofline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-frames", type=int, default=128)
+    ap.add_argument("--ra-chunk", type=int, default=0, help="fix the range-alignment chunk size (no autotuning)")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default range-alignment chunk size instead of autotuning it")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain iteration count")
+    ap.add_argument("--no-subapertures", action="store_true", help="skip the sub-aperture plan measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -167,6 +222,15 @@ def main() -> int:
     def step():
         im.run_device(echo.data_ptr(), vel.data_ptr(), img.data_ptr(), K, opts, stream=stream.cuda_stream)
 
+    # the autotuner picks the range-alignment chunk size on this batch (outside the timed region)
+    if args.ra_chunk:
+        im.set_ra_chunk(args.ra_chunk)
+        ra_chunk = args.ra_chunk
+    elif args.autotune:
+        ra_chunk = im.autotune(echo.data_ptr(), vel.data_ptr(), img.data_ptr(), K, opts, repeats=2,
+                               stream=stream.cuda_stream)
+    else:
+        ra_chunk = 8
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -219,7 +283,7 @@ def main() -> int:
         "config": {"workload": "256 pulses x 4096 range cells complex64 STRETCH frames, full plain pipeline "
                                "(HPC, range alignment, centring and cut, 120-iteration fast entropy, MTRC, "
                                "azimuth FFT)",
-                   "frames_per_step": K, "distinct_frames": D, "mean_selected_bins": U,
+                   "frames_per_step": K, "distinct_frames": D, "mean_selected_bins": U, "ra_chunk_rows": ra_chunk,
                    "wall_ms_timed_region": round(wall_ms, 3)},
         "roofline": {"bound": "hbm", "kernel": probe, "achieved": round(achieved_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 5),
@@ -231,11 +295,17 @@ def main() -> int:
                               "achieved_GBs": round(frame_bytes * fps / 1e9, 2),
                               "frac": round(frame_bytes * fps / 1e9 / HBM_PEAK_GBS, 5)},
     }
+    im.close()
+    del echo, vel, img
+    torch.cuda.empty_cache()
+    if not args.no_learned:
+        out["learned_vs_plain"] = learned_vs_plain()
+    if not args.no_subapertures:
+        out["subapertures"] = subapertures()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_frames)
         out["gpu_over_cpu"] = round(fps / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out))
-    im.close()
     return 0
 
 

@@ -114,6 +114,11 @@ ISAR_API int isar_run_frames(isar_context* ctx, int frames, const void* d_echo, 
  * choice changes the summation order of the group energies, not the algorithm. */
 ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows);
 ISAR_API int isar_get_ra_chunk(isar_context* ctx);
+/* Autotuner: times every range-alignment chunk size on these frames (`repeats` runs each after
+ * one warm-up, on `stream`), keeps the fastest for the context and returns it in *chosen.
+ * Synchronous; the image buffer receives the last run's images. */
+ISAR_API int isar_autotune(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity, float* d_img,
+                           const isar_options* opt, int repeats, int32_t* chosen, void* stream);
 
 /* Host batch: copies in, runs, copies the images out; synchronous. */
 ISAR_API int isar_run_host(isar_context* ctx, int frames, const float* h_echo, const double* h_velocity,

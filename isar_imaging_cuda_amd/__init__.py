@@ -214,6 +214,25 @@ class Imager:
         _check(lib().isar_get_phases(self._h, frames, C.c_void_p(start_ptr or 0), C.c_void_p(final_ptr or 0),
                                      C.c_void_p(stream or 0)), "isar_get_phases")
 
+    def set_ra_chunk(self, rows: int) -> None:
+        """Rows per workgroup in the range-alignment update (2, 4, 8 or 16)."""
+        L = lib()
+        L.isar_set_ra_chunk.argtypes = [C.c_void_p, C.c_int]
+        _check(L.isar_set_ra_chunk(self._h, int(rows)), "isar_set_ra_chunk")
+
+    def autotune(self, echo_ptr: int, velocity_ptr: int | None, img_ptr: int, frames: int,
+                 options: Options | None = None, repeats: int = 3, stream: int | None = None) -> int:
+        """Time every range-alignment chunk size on these frames, keep the fastest and return it."""
+        L = lib()
+        L.isar_autotune.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                    C.c_void_p, C.c_void_p]
+        chosen = C.c_int32(0)
+        o = (options or Options()).c()
+        _check(L.isar_autotune(self._h, frames, C.c_void_p(echo_ptr), C.c_void_p(velocity_ptr or 0),
+                               C.c_void_p(img_ptr), C.byref(o), repeats, C.byref(chosen), C.c_void_p(stream or 0)),
+               "isar_autotune")
+        return int(chosen.value)
+
     def probe_enable(self, stage: int) -> None:
         """Time every launch of kernel family `stage` (STAGES) with HIP events on the run stream."""
         _check(lib().isar_probe_enable(self._h, int(stage)), "isar_probe_enable")

@@ -289,6 +289,43 @@ int Context::run_frames(int frames, const void* echo, const double* velocity, co
     return ISAR_OK;
 }
 
+int Context::autotune(int frames, const void* echo, const double* velocity, const long long* frame_row, float* img,
+                      const long long* frame_img, const isar_options& opt, hipStream_t s, int repeats, int* chosen) {
+    if (repeats < 1) return fail(ISAR_ERR_INVALID, "autotune needs at least one timed run");
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+        return fail(ISAR_ERR_DEVICE, "autotune: events");
+    int rc = ISAR_OK;
+    float best_ms = 0.0f;
+    int best = ra_chunk;
+    for (int c : {2, 4, 8, 16}) {
+        if (c > p.echo_num / 2) continue;
+        ra_chunk = c;
+        rc = run_frames(frames, echo, velocity, frame_row, nullptr, img, frame_img, opt, s);   // warm-up
+        if (rc) break;
+        (void)hipEventRecord(a, s);
+        for (int r = 0; r < repeats && rc == ISAR_OK; ++r)
+            rc = run_frames(frames, echo, velocity, frame_row, nullptr, img, frame_img, opt, s);
+        (void)hipEventRecord(b, s);
+        if (rc) break;
+        if (hipEventSynchronize(b) != hipSuccess) {
+            rc = fail(ISAR_ERR_DEVICE, "autotune: timing");
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (best_ms == 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    ra_chunk = best;
+    if (chosen) *chosen = best;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return rc;
+}
+
 int Context::phases(int frames, void* d_start, void* d_final, hipStream_t s) {
     if (frames < 1 || frames > last_frames) return fail(ISAR_ERR_INVALID, "frames exceeds the last run's batch");
     const size_t bytes = static_cast<size_t>(frames) * p.echo_num * sizeof(float2);
@@ -442,6 +479,16 @@ ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows) {
 }
 
 ISAR_API int isar_get_ra_chunk(isar_context* ctx) { return ctx ? ctx->ra_chunk : 0; }
+
+ISAR_API int isar_autotune(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity, float* d_img,
+                           const isar_options* opt, int repeats, int32_t* chosen, void* stream) {
+    if (!ctx || !opt) return isar::fail(ISAR_ERR_INVALID, "null context or options");
+    int c = 0;
+    const int rc = ctx->autotune(frames, d_echo, d_velocity, nullptr, d_img, nullptr, *opt,
+                                 static_cast<hipStream_t>(stream), repeats, &c);
+    if (chosen) *chosen = c;
+    return rc;
+}
 
 ISAR_API int isar_get_phases(isar_context* ctx, int frames, void* d_phi_start, void* d_phi_final, void* stream) {
     if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");

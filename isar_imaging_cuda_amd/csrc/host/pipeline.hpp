@@ -54,6 +54,9 @@ struct Context {
                    const void* phi0, float* img, const long long* frame_img, const isar_options& opt,
                    hipStream_t stream);
     int ra_chunk = kRaChunkRowsDefault;   // tunable: rows per workgroup in the alignment update
+    // times every range-alignment chunk size on these frames and keeps the fastest (synchronous)
+    int autotune(int frames, const void* echo, const double* velocity, const long long* frame_row, float* img,
+                 const long long* frame_img, const isar_options& opt, hipStream_t s, int repeats, int* chosen);
     int trace(int frame, isar_frame_trace* out);
     int phases(int frames, void* d_start, void* d_final, hipStream_t s);
 };

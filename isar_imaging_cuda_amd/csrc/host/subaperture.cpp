@@ -227,42 +227,15 @@ ISAR_API int isar_sa_plan_autotune(isar_sa_plan* plan, const void* d_dwell, cons
     if (!plan || !d_dwell || !d_img || !opt || repeats < 1)
         return fail(ISAR_ERR_INVALID, "isar_sa_plan_autotune: bad arguments");
     plan->drop_graph();
-    hipEvent_t a = nullptr, b = nullptr;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
-        return fail(ISAR_ERR_DEVICE, "isar_sa_plan_autotune: events");
-    int rc = ISAR_OK;
-    const int candidates[] = {2, 4, 8, 16};
-    for (size_t gi = 0; gi < plan->groups.size() && rc == ISAR_OK; ++gi) {
+    for (size_t gi = 0; gi < plan->groups.size(); ++gi) {
         isar::SaGroup* g = plan->groups[gi];
-        const int K = static_cast<int>(g->windows.size());
-        float best_ms = 0.0f;
-        int best = g->ctx.ra_chunk;
-        for (int c : candidates) {
-            if (c > g->M / 2) continue;
-            g->ctx.ra_chunk = c;
-            rc = g->ctx.run_frames(K, d_dwell, d_velocity, g->d_rows, nullptr, d_img, g->d_img, *opt, plan->stream);
-            if (rc) break;
-            (void)hipEventRecord(a, plan->stream);
-            for (int r = 0; r < repeats && rc == ISAR_OK; ++r)
-                rc = g->ctx.run_frames(K, d_dwell, d_velocity, g->d_rows, nullptr, d_img, g->d_img, *opt, plan->stream);
-            (void)hipEventRecord(b, plan->stream);
-            if (rc || hipEventSynchronize(b) != hipSuccess) {
-                if (!rc) rc = fail(ISAR_ERR_DEVICE, "isar_sa_plan_autotune: timing");
-                break;
-            }
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, a, b);
-            if (best_ms == 0.0f || ms < best_ms) {
-                best_ms = ms;
-                best = c;
-            }
-        }
-        g->ctx.ra_chunk = best;
-        if (chosen) chosen[gi] = best;
+        int c = 0;
+        const int rc = g->ctx.autotune(static_cast<int>(g->windows.size()), d_dwell, d_velocity, g->d_rows, d_img,
+                                       g->d_img, *opt, plan->stream, repeats, &c);
+        if (rc) return rc;
+        if (chosen) chosen[gi] = c;
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    return rc;
+    return ISAR_OK;
 }
 
 }  // extern "C"
