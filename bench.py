@@ -184,8 +184,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-frames", type=int, default=128)
     ap.add_argument("--ra-chunk", type=int, default=0, help="fix the range-alignment chunk size (no autotuning)")
-    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
-                    help="keep the default range-alignment chunk size instead of autotuning it")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let the autotuner pick the range-alignment chunk size on this batch (default: 8 rows, "
+                         "the size the committed traffic measurement used)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain iteration count")
     ap.add_argument("--no-subapertures", action="store_true", help="skip the sub-aperture plan measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -222,7 +223,7 @@ def main() -> int:
     def step():
         im.run_device(echo.data_ptr(), vel.data_ptr(), img.data_ptr(), K, opts, stream=stream.cuda_stream)
 
-    # the autotuner picks the range-alignment chunk size on this batch (outside the timed region)
+    # optionally the autotuner picks the range-alignment chunk size on this batch (outside the timed region)
     if args.ra_chunk:
         im.set_ra_chunk(args.ra_chunk)
         ra_chunk = args.ra_chunk

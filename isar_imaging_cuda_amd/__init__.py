@@ -55,12 +55,28 @@ STAGES = {"ifds_pre": 0, "ra_pair0": 1, "ra_corr": 2, "ra_upd": 3, "center": 4, 
 _lib = None
 
 
+def _torch_runtime_first() -> None:
+    """PyTorch's ROCm wheel carries its own HIP and HSA runtimes next to the system ones this
+    library links. Two runtimes coexist in one process when PyTorch's initialises first; the
+    other order leaves PyTorch without a GPU. When PyTorch is importable, initialise it first."""
+    try:
+        import torch
+    except Exception:
+        return
+    try:
+        if torch.cuda.is_available():
+            torch.cuda.init()
+    except Exception:
+        pass
+
+
 def lib() -> C.CDLL:
     """Load the HIP library (raises if it has not been built)."""
     global _lib
     if _lib is None:
         if not library_path.exists():
             raise ImagingError(f"{library_path} is missing: run __graft_entry__.build() first")
+        _torch_runtime_first()
         L = C.CDLL(str(library_path))
         vp = C.c_void_p
         L.isar_default_options.argtypes = [C.POINTER(_Options)]
