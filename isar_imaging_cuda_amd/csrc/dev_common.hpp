@@ -583,12 +583,14 @@ namespace isar {
 // Sum over aligned groups of WIDTH lanes (WIDTH a power of two <= 64).
 template <int WIDTH>
 __device__ __forceinline__ float group_sum(float v) {
+    static_assert(WIDTH <= 64, "shuffle groups cannot span waves");
 #pragma unroll
     for (int o = WIDTH / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WIDTH);
     return v;
 }
 template <int WIDTH>
 __device__ __forceinline__ float2 group_sum2(float2 v) {
+    static_assert(WIDTH <= 64, "shuffle groups cannot span waves");
 #pragma unroll
     for (int o = WIDTH / 2; o > 0; o >>= 1) {
         v.x += __shfl_xor(v.x, o, WIDTH);

@@ -142,8 +142,21 @@ __global__ __launch_bounds__(256) void k_mtrc_czt(Job J) {
             csum = cadd(csum, y[e]);
         }
     }
-    csum = group_sum2<TPF>(csum);
-    if (lane == 0) J.colsum[static_cast<size_t>(f) * W + n] = csum;
+    if constexpr (TPF <= 64) {
+        csum = group_sum2<TPF>(csum);
+        if (lane == 0) J.colsum[static_cast<size_t>(f) * W + n] = csum;
+    } else {
+        // one transform spans TPF / 64 waves: wave sums first, then the transform's waves in LDS
+        __shared__ float2 wave_part[NT / 64];
+        csum = group_sum2<64>(csum);
+        if ((tid & 63) == 0) wave_part[tid / 64] = csum;
+        __syncthreads();
+        if (lane == 0) {
+            float2 t = make_float2(0.0f, 0.0f);
+            for (int w = 0; w < TPF / 64; ++w) t = cadd(t, wave_part[g * (TPF / 64) + w]);
+            J.colsum[static_cast<size_t>(f) * W + n] = t;
+        }
+    }
     // transpose back through LDS for row-contiguous stores
 #pragma unroll
     for (int e = 0; e < E; ++e) {

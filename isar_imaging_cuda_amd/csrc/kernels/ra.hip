@@ -147,7 +147,8 @@ __device__ float correlate_pair(const float (&a)[kE], const float (&b)[kE], floa
             if (fa != 0.0f) delta = -fb / (2.0f * fa);
             if (!(delta == delta) || fabsf(delta) > 1e30f) delta = 0.0f;
         }
-        const float lag = static_cast<float>(bi) + delta - static_cast<float>(N) / 2.0f;
+        // integer part first: bi + delta rounded at bi's magnitude would lose ~N/2^24 cells
+        const float lag = static_cast<float>(bi - N / 2) + delta;
         *lag_out = lag;
         *imax_out = bi;
         redv[32] = lag;

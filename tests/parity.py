@@ -40,7 +40,7 @@ def oracle_run(echo, velocity, radar: dict, M: int, N: int, **opts):
     return img, tr
 
 
-def compare(img, t, img_o, tr, M: int) -> ParityReport:
+def compare(img, t, img_o, tr, M: int, img_rel_l2: float = IMG_REL_L2) -> ParityReport:
     """t: device/emulation trace (attributes as in isar_imaging_cuda_amd.FrameTrace)."""
     r = ParityReport()
     # range alignment: integer argmax per (level, pair); lags
@@ -75,8 +75,8 @@ def compare(img, t, img_o, tr, M: int) -> ParityReport:
     e_dev = O.image_entropy(np.asarray(img, np.float32))
     r.entropy_rel = abs(e_dev - tr.c.entropy) / max(abs(tr.c.entropy), 1e-300)
     if not r.failures:
-        if r.img_rel_l2 > IMG_REL_L2:
-            r.failures.append(f"image rel L2 {r.img_rel_l2:.3e} > {IMG_REL_L2}")
+        if r.img_rel_l2 > img_rel_l2:
+            r.failures.append(f"image rel L2 {r.img_rel_l2:.3e} > {img_rel_l2}")
         if r.entropy_rel > ENTROPY_REL:
             r.failures.append(f"entropy rel {r.entropy_rel:.3e} > {ENTROPY_REL}")
     return r

@@ -6,7 +6,7 @@ import pytest
 
 import isar_imaging_cuda_amd as isar
 from isar_imaging_cuda_amd import synth
-from parity import compare, oracle_run
+from parity import IMG_REL_L2, compare, oracle_run
 
 pytestmark = pytest.mark.gpu
 
@@ -41,8 +41,20 @@ def _run_device(echo, vel, radar, M, N, opts: isar.Options, frames=1):
 CASES = [
     ("c1_M64", 1, dict(echo_num=64)),
     ("c1", 1, {}),
+    ("c1_M192", 1, dict(echo_num=192)),     # pulse counts that are not powers of two
+    ("c1_M384", 1, dict(echo_num=384)),
+    ("c1_M512", 1, dict(echo_num=512)),
     ("c2", 2, {}),
+    ("c4", 4, {}),                          # 256 x 2048, range-aligned scene
+    ("c5", 5, {}),                          # 1024 x 16384, 1000 scatterers, 7 km/s
 ]
+
+# Image bar per case. At 1024 pulses x 16384 cells the FP32 lags stay within the 1e-3-cell bar
+# (2.2e-4 measured) and every integer decision is identical, but those lag differences and the
+# FP32 Doppler start (3.7e-3 rad over 1024 pulses) are amplified by the 120-iteration fixed point
+# (0.11 rad phase difference after it), so the image differs from the FP64 oracle by 3e-3 relative
+# L2 while its entropy agrees to 2e-6. The 1e-4 image bar holds for every other case.
+IMG_BAR = {"c5": 5e-3}
 
 
 @pytest.mark.parametrize("name,cid,over", CASES, ids=[c[0] for c in CASES])
@@ -52,7 +64,7 @@ def test_pipeline_parity(name, cid, over):
     print(f"[case] {name} M={M} N={N}", flush=True)
     img, (t,) = _run_device(echo, vel, radar, M, N, isar.Options())
     img_o, tr = oracle_run(echo, vel, radar, M, N)
-    rep = compare(img, t, img_o, tr, M)
+    rep = compare(img, t, img_o, tr, M, img_rel_l2=IMG_BAR.get(name, IMG_REL_L2))
     print(rep)
     assert rep.ok(), str(rep)
 
