@@ -94,7 +94,9 @@ size_t workspace_bytes(const isar_radar_params& p, int K) {
     per += 4 * M * W * sizeof(float2);          // x, stm, czt, su
     per += M * sizeof(float2) * 3;              // rowdop, phi0, phi
     per += nbands * W * sizeof(float) + 3 * W * sizeof(float) + W * sizeof(int32_t) + W * sizeof(float2);
-    return per * K + W * F * sizeof(float2);
+    per += W * sizeof(float2);                  // delta
+    // shared: chirp-z filter spectra, input/output chirps, range window
+    return per * K + W * F * sizeof(float2) + 2 * W * M * sizeof(float2) + N * sizeof(float);
 }
 
 Context::~Context() {
@@ -188,6 +190,9 @@ int Context::init(const isar_radar_params& params, int frames) {
     J.czt = static_cast<float2*>(alloc(K * M * W * sizeof(float2)));
     J.colsum = static_cast<float2*>(alloc(K * W * sizeof(float2)));
     czt_table = static_cast<float2*>(alloc(W * F * sizeof(float2)));
+    float2* czt_pre = static_cast<float2*>(alloc(W * M * sizeof(float2)));
+    float2* czt_post = static_cast<float2*>(alloc(W * M * sizeof(float2)));
+    J.delta = static_cast<float2*>(alloc(K * W * sizeof(float2)));
     float* win_range = static_cast<float*>(alloc(N * sizeof(float)));
     if (!ok) return fail(ISAR_ERR_MEMORY, "device allocation failed for " + std::to_string(frames) + " frames");
     // constants (high_speed_compensation.cu:16-18, mtrc.cu:10-12, 47)
@@ -205,6 +210,9 @@ int Context::init(const isar_radar_params& params, int frames) {
         J.win_range = win_range;
     }
     launch_czt_table(J.M, J.W, F, J.czt_c0, J.czt_posa, czt_table, nullptr);
+    launch_czt_chirps(J.M, J.W, F, J.czt_c0, J.czt_posa, czt_pre, czt_post, nullptr);
+    J.czt_pre = czt_pre;
+    J.czt_post = czt_post;
     hipError_t e = hipDeviceSynchronize();
     if (e == hipSuccess) e = hipGetLastError();
     if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("chirp-z table: ") + hipGetErrorString(e));
@@ -278,6 +286,7 @@ int Context::run_frames(int frames, const void* echo, const double* velocity, co
         probed(ISAR_STAGE_MTRC_ROWS, s, [&] { launch_mtrc_rows(J, s); });
         probed(ISAR_STAGE_MTRC_CZT, s, [&] { launch_mtrc_czt(J, s); });
         if (!J.quirk_q1) probed(ISAR_STAGE_COL_WINDOW, s, [&] { launch_col_window(J, s); });
+        else probed(ISAR_STAGE_MTRC_IMAGE, s, [&] { launch_mtrc_delta(J, s); });
         probed(ISAR_STAGE_MTRC_IMAGE, s, [&] { launch_mtrc_image(J, s); });
     } else {
         probed(ISAR_STAGE_POST_NOMTRC, s, [&] { launch_post_nomtrc(J, s); });

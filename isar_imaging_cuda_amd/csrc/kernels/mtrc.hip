@@ -64,6 +64,17 @@ __global__ __launch_bounds__(F / 16) void k_czt_table(int M, int W, double c0, d
     for (int e = 0; e < C::E; ++e) table[static_cast<size_t>(n) * F + C::out(lane, e)] = x[e];
 }
 
+// Input and output chirps of the chirp-z transform per range bin n (mtrc.cu:125-192), phases
+// reduced in FP64: pre[n][m] = exp(j pi a_n m (M - m) / M), post[n][k] = exp(-j pi a_n k^2 / M) / F.
+__global__ void k_czt_chirps(int M, int W, int F, double c0, double posa, float2* pre, float2* post) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M * W) return;
+    const int n = i / M, m = i % M;
+    const double alpha = 1.0 - c0 + posa * n;
+    pre[i] = expjpi_reduced(alpha * static_cast<double>(m) * (M - m) / M);
+    post[i] = cscale(expjpi_reduced(-alpha * static_cast<double>(m) * m / M), 1.0f / static_cast<float>(F));
+}
+
 // Range-frequency rows (mtrc.cu:14-40) with the autofocus phase applied (phase_adjustment.cu:324).
 template <int W>
 __global__ __launch_bounds__(kRowsPerBlock * W / kRowE) void k_mtrc_rows(Job J) {
@@ -114,23 +125,23 @@ __global__ __launch_bounds__(256) void k_mtrc_czt(Job J) {
         sm[m * (CT + 1) + c] = S[static_cast<size_t>(m) * W + n0 + c];
     }
     __syncthreads();
-    const double alpha = 1.0 - J.czt_c0 + J.czt_posa * n;
+    const float2* pre = J.czt_pre + static_cast<size_t>(n) * M;
+    const float2* post = J.czt_post + static_cast<size_t>(n) * M;
     float2 x[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int pos = C::in(lane, e);
-        x[e] = pos < M ? cmul(sm[pos * (CT + 1) + g], expjpi_reduced(alpha * static_cast<double>(pos) * (M - pos) / M))
-                       : make_float2(0.0f, 0.0f);
+        x[e] = pos < M ? cmul(sm[pos * (CT + 1) + g], pre[pos]) : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
     float2* buf = sm + g * lds_padded_size(F);
-    fft<F, E, -1>(x, buf, lane);
+    const FftTwiddles<F, E> tw = fft_twiddles<F, E>(lane);
+    fft<F, E, -1>(x, buf, lane, tw);
     const float2* tab = J.czt_table + static_cast<size_t>(n) * F;
 #pragma unroll
     for (int e = 0; e < E; ++e) x[e] = cmul(x[e], tab[C::out(lane, e)]);
     relayout_out_to_in<F, E>(x, buf, lane);
-    fft<F, E, +1>(x, buf, lane);
-    const float invF = 1.0f / F;
+    fft<F, E, +1>(x, buf, lane, tw);
     float2 csum = make_float2(0.0f, 0.0f);
     float2 y[E];
 #pragma unroll
@@ -138,7 +149,7 @@ __global__ __launch_bounds__(256) void k_mtrc_czt(Job J) {
         const int k = C::out(lane, e);
         y[e] = make_float2(0.0f, 0.0f);
         if (k < M) {
-            y[e] = cscale(cmul(x[e], expjpi_reduced(-alpha * static_cast<double>(k) * k / M)), invF);
+            y[e] = cmul(x[e], post[k]);
             csum = cadd(csum, y[e]);
         }
     }
@@ -146,8 +157,10 @@ __global__ __launch_bounds__(256) void k_mtrc_czt(Job J) {
         csum = group_sum2<TPF>(csum);
         if (lane == 0) J.colsum[static_cast<size_t>(f) * W + n] = csum;
     } else {
-        // one transform spans TPF / 64 waves: wave sums first, then the transform's waves in LDS
-        __shared__ float2 wave_part[NT / 64];
+        // one transform spans TPF / 64 waves: wave sums first, then the transform's waves in LDS.
+        // The partials sit past the transpose area (M (CT + 1) entries), which the dynamic LDS
+        // size leaves free whenever a transform spans several waves (CT <= 2).
+        float2* wave_part = sm + static_cast<size_t>(M) * (CT + 1);
         csum = group_sum2<64>(csum);
         if ((tid & 63) == 0) wave_part[tid / 64] = csum;
         __syncthreads();
@@ -192,18 +205,42 @@ __global__ __launch_bounds__(256) void k_col_window(Job J) {
     for (int e = 0; e < E; ++e) x[e] = sm[((C::in(lane, e) + M / 2) % M) * (CT + 1) + g];
     __syncthreads();
     float2* buf = sm + g * lds_padded_size(M);
-    fft<M, E, +1>(x, buf, lane);
+    const FftTwiddles<M, E> tw = fft_twiddles<M, E>(lane);
+    fft<M, E, +1>(x, buf, lane, tw);
     const float invM = 1.0f / M;
 #pragma unroll
     for (int e = 0; e < E; ++e) x[e] = cscale(x[e], invM * hamming(C::out(lane, e), M));
     relayout_out_to_in<M, E>(x, buf, lane);
-    fft<M, E, -1>(x, buf, lane);
+    fft<M, E, -1>(x, buf, lane, tw);
 #pragma unroll
     for (int e = 0; e < E; ++e) sm[((C::out(lane, e) + M / 2) % M) * (CT + 1) + g] = x[e];
     __syncthreads();
     for (int idx = tid; idx < M * CT; idx += NT) {
         const int m = idx / CT, c = idx % CT;
         D[static_cast<size_t>(m) * W + n0 + c] = sm[m * (CT + 1) + c];
+    }
+}
+
+// Slow-time window quirk term, once per frame: delta = (c - 1) / M * ([L] P_W F_W colsum),
+// stored at the last-pass positions of the row transform.
+template <int W>
+__global__ __launch_bounds__(W / kRowE) void k_mtrc_delta(Job J) {
+    HIP_DYNAMIC_SHARED(float2, buf);
+    const int lane = threadIdx.x, f = blockIdx.x;
+    const int M = J.M;
+    const bool ifds = J.data_type == 1;
+    const float2* cs = J.colsum + static_cast<size_t>(f) * W;
+    float2 d[kRowE];
+#pragma unroll
+    for (int e = 0; e < kRowE; ++e) d[e] = cs[row_in<W>(lane, e)];
+    fft<W, kRowE, -1>(d, buf, lane);
+    float2* out = J.delta + static_cast<size_t>(f) * W;
+#pragma unroll
+    for (int e = 0; e < kRowE; ++e) {
+        const int p = row_out<W>(lane, e);
+        int j = (p + W / 2) & (W - 1);
+        if (ifds) j = W - 1 - j;
+        out[p] = cscale(d[e], (hamming(j / M, M) - 1.0f) / static_cast<float>(M));
     }
 }
 
@@ -222,16 +259,9 @@ __global__ __launch_bounds__(kRowsPerBlock * W / kRowE) void k_mtrc_image(Job J)
     float2 d[kRowE];
     const bool quirk = J.quirk_q1 != 0;
     if (quirk) {
-        const float2* cs = J.colsum + static_cast<size_t>(f) * W;
+        const float2* dl = J.delta + static_cast<size_t>(f) * W;
 #pragma unroll
-        for (int e = 0; e < kRowE; ++e) d[e] = cs[row_in<W>(lane, e)];
-        fft<W, kRowE, -1>(d, buf, lane);
-#pragma unroll
-        for (int e = 0; e < kRowE; ++e) {
-            int j = (row_out<W>(lane, e) + W / 2) & (W - 1);
-            if (ifds) j = W - 1 - j;
-            d[e] = cscale(d[e], (hamming(j / M, M) - 1.0f) / static_cast<float>(M));
-        }
+        for (int e = 0; e < kRowE; ++e) d[e] = dl[row_out<W>(lane, e)];
     }
     const int kk = valid ? (ifds ? M - 1 - k : k) : 0;
     const float2* cr = J.czt + (static_cast<size_t>(f) * M + kk) * W;
@@ -358,6 +388,13 @@ hipError_t configure_mtrc_kernels() {
     return err;
 }
 
+void launch_czt_chirps(int M, int W, int F, double c0, double posa, float2* pre, float2* post, hipStream_t s) {
+    ISAR_LAUNCH(k_czt_chirps, dim3((M * W + 255) / 256), dim3(256), 0, s, M, W, F, c0, posa, pre, post);
+}
+void launch_mtrc_delta(const Job& J, hipStream_t s) {
+    ISAR_LAUNCH(k_mtrc_delta<kRangeCut>, dim3(J.K), dim3(kRangeCut / kRowE), lds_padded_size(kRangeCut) * sizeof(float2),
+                s, J);
+}
 void launch_czt_table(int M, int W, int F, double c0, double posa, float2* table, hipStream_t s) {
     ISAR_F_SWITCH(F, ISAR_LAUNCH(k_czt_table<FF>, dim3(W), dim3(FF / 16), lds_padded_size(FF) * sizeof(float2), s,
                                  M, W, c0, posa, table));

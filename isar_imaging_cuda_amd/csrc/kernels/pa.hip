@@ -216,6 +216,7 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     float2* buf = bufs + g * lds_padded_size(M);
     const float invM = 1.0f / static_cast<float>(M);
     const bool resident = U <= CF;   // one chunk: the columns stay in registers
+    const FftTwiddles<M, E> tw = fft_twiddles<M, E>(lane);
     float2 s[E];
     for (int it = 0; it < J.iterations; ++it) {
         float2 acc[RPT];
@@ -232,11 +233,11 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
             float2 x[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) x[e] = cmul(phi[C::in(lane, e)], s[e]);
-            fft<M, E, -1>(x, buf, lane);
+            fft<M, E, -1>(x, buf, lane, tw);
 #pragma unroll
             for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabsf_(x[e]));
             relayout_out_to_in<M, E>(x, buf, lane);
-            fft<M, E, +1>(x, buf, lane);
+            fft<M, E, +1>(x, buf, lane, tw);
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int pos = C::out(lane, e);

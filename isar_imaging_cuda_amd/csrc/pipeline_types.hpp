@@ -75,6 +75,9 @@ struct Job {
     float2* czt;                  // K x M x W (chirp-z output)
     float2* colsum;               // K x W
     const float2* czt_table;      // W x F (FFT of the Bluestein chirp per range bin)
+    const float2* czt_pre;        // W x M: exp(j pi a_n m (M - m) / M), the input chirp
+    const float2* czt_post;       // W x M: exp(-j pi a_n k^2 / M) / F, the output chirp
+    float2* delta;                // K x W: transformed column sums for the slow-time window quirk
     const float* win_range;       // N: Hamming window over range cells (genHammingVec, common.cu:260-266)
     // output
     float* img;                   // K x M x W
