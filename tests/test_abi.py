@@ -53,8 +53,10 @@ def test_invalid_sizes_return_error_codes():
 
 def test_workspace_bytes_scales_with_frames():
     p = isar.RadarParams(256, 4096)
-    one, eight = isar.Imager.workspace_bytes(p, 1), isar.Imager.workspace_bytes(p, 8)
-    assert one > 256 * 4096 * 8 and eight > 7 * one
+    one, two, eight = (isar.Imager.workspace_bytes(p, k) for k in (1, 2, 8))
+    per_frame = two - one                      # frame-sized buffers; the rest is shared tables
+    assert per_frame > 256 * 4096 * 8 and one > per_frame
+    assert eight == one + 7 * per_frame
 
 
 def test_wbd_ingest_decodes_headers(tmp_path):
