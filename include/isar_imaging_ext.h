@@ -110,6 +110,17 @@ ISAR_API int isar_run_frames(isar_context* ctx, int frames, const void* d_echo, 
                              const int64_t* d_frame_rows, const void* d_phi0, float* d_img,
                              const int64_t* d_frame_img, const isar_options* opt, void* stream);
 
+/* IFDS pulse compression (common.cu:583-604, 635-746 of the reference): `echoes` raw echoes of
+ * `len` int16 I/Q pairs each (d_iq, as read from IFDS .wbd frames after the 256-byte header),
+ * one velocity per echo (m/s). Each echo is matched-filtered with the velocity-dependent
+ * reference chirp (Fs Tp samples, Hamming-weighted) through NFFT = nextPow2(len)-point
+ * transforms (nextPow2 as in common.cu:521-529: 2 len for powers of two; NFFT 4096, 8192 or
+ * 16384), and the 4096 samples around the first maximum of |Re| + |Im| are written to
+ * d_out + 4096 e (complex float). d_offsets (optional) receives each cut offset.
+ * Asynchronous on `stream`. */
+ISAR_API int isar_pulse_compress(const isar_radar_params* p, int echoes, int len, const int16_t* d_iq,
+                                 const double* d_velocity, void* d_out, int32_t* d_offsets, void* stream);
+
 /* Rows of the range-alignment update handled per workgroup (2, 4, 8 or 16; default 8). The
  * choice changes the summation order of the group energies, not the algorithm. */
 ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows);

@@ -53,7 +53,8 @@ typedef struct isar_scene_config {
 } isar_scene_config;
 
 /* Fill `cfg` with a preset: 1..5 are the five benchmark configurations (sizes, scene, motion,
- * phase error and SNR), 4 uses the range-aligned 256 x 2048 learned-autofocus frames.
+ * phase error and SNR), 4 uses the range-aligned 256 x 2048 learned-autofocus frames, 6 is an
+ * IFDS (raw chirp) configuration for isar_synth_ifds.
  * Returns 0, or -1 for an unknown id. */
 int isar_synth_preset(int config_id, isar_scene_config* cfg);
 
@@ -61,6 +62,15 @@ int isar_synth_preset(int config_id, isar_scene_config* cfg);
  * truth_phase (M) and truth_walk (M, metres) may be NULL. Returns 0 or -1. */
 int isar_synth_frame(const isar_scene_config* cfg, float* echo, double* velocity,
                      double* truth_phase, double* truth_walk);
+
+/* Raw IFDS echoes (intermediate-frequency direct sampling): M pulses of `len` complex samples
+ * (iq: 2*M*len floats, int16-valued when quantised) at cfg->fs, each the sum over scatterers of
+ * the linear chirp of cfg->tp seconds and cfg->band_width sweep that the reference's pulse
+ * compression correlates against (common.cu:635-746, with the pulse's velocity), delayed to the
+ * scatterer's range (nearest sample, c / (2 fs) metres per sample) and rotated by its carrier
+ * phase -4 pi fc dR / c plus the per-pulse phase error. velocity: M doubles. Preset 6 is an
+ * IFDS configuration (range_num = len = 6000). Returns 0 or -1. */
+int isar_synth_ifds(const isar_scene_config* cfg, int len, float* iq, double* velocity);
 
 /* Radar parameters that go with a config, in the reference's integer header units
  * (MHz and microseconds, common.cu:809-826). */

@@ -2,7 +2,7 @@
 // the drop-in library: gpuDevInit -> dataParsing -> imagingMemInit -> { dataExtracting ->
 // isarMainSingle } x n -> imagingMemDest. Built against include/isar_imaging_export.h only.
 //
-// usage: isar_calling_app <dir> <window_len> <imaging_stride> <n_images> <out_prefix>
+// usage: isar_calling_app <dir> <window_len> <imaging_stride> <n_images> <out_prefix> [polar] [mtrc] [hpc] [data_type]
 //                         [polar_type=1] [if_mtrc=1] [if_hpc=1]
 // writes <out_prefix>_<i>.bin (raw float32 image, window_len x 512) and <out_prefix>_0.txt.
 // n_images = 0 only parses the directory (no GPU call) and prints the decoded headers.
@@ -17,7 +17,8 @@
 
 int main(int argc, char** argv) {
     if (argc < 6) {
-        std::fprintf(stderr, "usage: %s dir window_len imaging_stride n_images out_prefix [polar] [mtrc] [hpc]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s dir window_len imaging_stride n_images out_prefix [polar] [mtrc] [hpc] [data_type]\n",
+                     argv[0]);
         return 2;
     }
     const std::string dir_path = argv[1];
@@ -28,7 +29,7 @@ int main(int argc, char** argv) {
     const int polar_type = argc > 6 ? std::atoi(argv[6]) : 1;
     const bool if_mtrc = argc > 7 ? std::atoi(argv[7]) != 0 : true;
     const bool if_hpc = argc > 8 ? std::atoi(argv[8]) != 0 : true;
-    const int data_type = 2;          // STRETCH
+    const int data_type = argc > 9 ? std::atoi(argv[9]) : 2;   // 1 IFDS, 2 STRETCH
     const int sampling_stride = 1;
     int window_head = 0;
     const int option_alignment = 0, option_phase = 1;

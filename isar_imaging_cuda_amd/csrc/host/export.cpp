@@ -44,6 +44,10 @@ struct Session {
     double* d_velocity = nullptr;
     float* d_img = nullptr;
     std::vector<double> h_velocity;
+    // IFDS: raw echoes staged for pulse compression (common.cu:583-604)
+    int16_t* d_iq = nullptr;
+    double* d_pc_velocity = nullptr;
+    int iq_len = 0;
 };
 Session g;
 
@@ -160,11 +164,15 @@ void free_buffers() {
     if (g.ctx) isar_destroy(g.ctx);
     g.ctx = nullptr;
     for (void* p : {static_cast<void*>(g.d_pp[0]), static_cast<void*>(g.d_pp[1]), static_cast<void*>(g.d_proc),
-                    static_cast<void*>(g.d_velocity), static_cast<void*>(g.d_img)})
+                    static_cast<void*>(g.d_velocity), static_cast<void*>(g.d_img), static_cast<void*>(g.d_iq),
+                    static_cast<void*>(g.d_pc_velocity)})
         if (p) (void)hipFree(p);
     g.d_pp[0] = g.d_pp[1] = g.d_proc = nullptr;
     g.d_velocity = nullptr;
     g.d_img = nullptr;
+    g.d_iq = nullptr;
+    g.d_pc_velocity = nullptr;
+    g.iq_len = 0;
     g.active = -1;
 }
 
@@ -230,6 +238,11 @@ DLL_EXPORT_API void imagingMemInit(vec1D_FLT* img, vec1D_INT* dataWFileSn, vec1D
               hipMalloc(&g.d_proc, M * N * sizeof(float2)) == hipSuccess &&
               hipMalloc(&g.d_velocity, M * sizeof(double)) == hipSuccess &&
               hipMalloc(&g.d_img, M * W * sizeof(float)) == hipSuccess;
+    if (ok && data_type == kIfds) {
+        g.iq_len = (frame_len - 256) / 4;
+        ok = hipMalloc(&g.d_iq, M * static_cast<size_t>(g.iq_len) * 2 * sizeof(int16_t)) == hipSuccess &&
+             hipMalloc(&g.d_pc_velocity, M * sizeof(double)) == hipSuccess;
+    }
     if (!ok) {
         free_buffers();
         err("device allocation failed in imagingMemInit");
@@ -244,7 +257,8 @@ DLL_EXPORT_API int dataExtracting(vec1D_INT* dataWFileSn, vec1D_DBL* dataNOut, v
                                   const int& frame_num, const int& sampling_stride, const int& window_head,
                                   const int& window_len, int& imaging_stride, const int& data_type) {
     if (!g.ctx) return err("dataExtracting before imagingMemInit"), EXIT_FAILURE;
-    if (data_type != kStretch) return err("IFDS ingest (pulse compression) is not provided"), EXIT_FAILURE;
+    if (data_type != kStretch && data_type != kIfds) return err("unknown data type"), EXIT_FAILURE;
+    if (data_type == kIfds && !g.d_iq) return err("IFDS data after an imagingMemInit for STRETCH"), EXIT_FAILURE;
     const int M = g.paras.echo_num, N = g.paras.range_num;
     if (window_len != M) return err("window_len differs from imagingMemInit"), EXIT_FAILURE;
     // uniformSampling (common.cu:1165-1185)
@@ -271,8 +285,11 @@ DLL_EXPORT_API int dataExtracting(vec1D_INT* dataWFileSn, vec1D_DBL* dataNOut, v
     hipError_t e = hipSuccess;
     if (overlap > 0)
         e = hipMemcpy(cur, g.d_pp[old] + static_cast<size_t>(fresh) * N, sizeof(float2) * overlap * N, hipMemcpyDeviceToDevice);
-    // STRETCH read (common.cu:1082-1104): int16 I/Q pairs after the 256-byte frame header
+    for (int i = 0; i < M; ++i) g.h_velocity[i] = (*dataNOut)[static_cast<size_t>(M) + i];
+    // int16 I/Q pairs after the 256-byte frame header (common.cu:1044-1100)
     std::vector<int16_t> ad(static_cast<size_t>(frame_len - 256) / 2);
+    std::vector<int16_t> iq;   // IFDS: the fresh echoes, raw
+    if (data_type == kIfds) iq.resize(static_cast<size_t>(fresh) * ad.size());
     std::vector<std::ifstream> fin(g.files.size());
     for (int i = overlap; i < M && e == hipSuccess; ++i) {
         const int sn = (*dataWFileSn)[i];
@@ -281,13 +298,29 @@ DLL_EXPORT_API int dataExtracting(vec1D_INT* dataWFileSn, vec1D_DBL* dataNOut, v
         fin[fi].seekg(static_cast<std::streamoff>(sn - fi * frame_num) * frame_len + 256);
         fin[fi].read(reinterpret_cast<char*>(ad.data()), static_cast<std::streamsize>(ad.size() * sizeof(int16_t)));
         if (!fin[fi]) return err("short read in " + g.files[fi]), EXIT_FAILURE;
-        std::complex<float>* row = dataW->data() + static_cast<size_t>(i) * N;
-        for (size_t j = 0; j + 1 < ad.size(); j += 2) row[j / 2] = {static_cast<float>(ad[j]), static_cast<float>(ad[j + 1])};
+        if (data_type == kIfds) {
+            std::copy(ad.begin(), ad.end(), iq.begin() + static_cast<size_t>(i - overlap) * ad.size());
+        } else {
+            std::complex<float>* row = dataW->data() + static_cast<size_t>(i) * N;
+            for (size_t j = 0; j + 1 < ad.size(); j += 2) row[j / 2] = {static_cast<float>(ad[j]), static_cast<float>(ad[j + 1])};
+        }
     }
-    if (e == hipSuccess && fresh > 0)
+    if (data_type == kIfds && e == hipSuccess && fresh > 0) {
+        // pulse compression on the device (multiThreadIFDS, common.cu:583-604). The reference
+        // takes echo i's velocity from dataNOut[M + dataWFileSn[i]], an index into the window's
+        // 4 x M table by absolute frame number (out of range once the window moves); here it is
+        // echo i's own velocity, dataNOut[M + i].
+        e = hipMemcpy(g.d_iq, iq.data(), iq.size() * sizeof(int16_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(g.d_pc_velocity, g.h_velocity.data() + overlap, sizeof(double) * fresh, hipMemcpyHostToDevice);
+        if (e == hipSuccess &&
+            isar_pulse_compress(&g.paras, fresh, g.iq_len, g.d_iq, g.d_pc_velocity, cur + static_cast<size_t>(overlap) * N,
+                                nullptr, nullptr) != ISAR_OK)
+            return EXIT_FAILURE;   // reason recorded by isar_pulse_compress
+    } else if (e == hipSuccess && fresh > 0) {
         e = hipMemcpy(cur + static_cast<size_t>(overlap) * N, dataW->data() + static_cast<size_t>(overlap) * N,
                       sizeof(float2) * fresh * N, hipMemcpyHostToDevice);
-    for (int i = 0; i < M; ++i) g.h_velocity[i] = (*dataNOut)[static_cast<size_t>(M) + i];
+    }
     if (e == hipSuccess) e = hipMemcpy(g.d_velocity, g.h_velocity.data(), sizeof(double) * M, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(g.d_proc, cur, sizeof(float2) * static_cast<size_t>(M) * N, hipMemcpyDeviceToDevice);
     if (e != hipSuccess) return err(std::string("dataExtracting copy: ") + hipGetErrorString(e)), EXIT_FAILURE;

@@ -14,6 +14,7 @@ hipError_t configure_ra_kernels();
 hipError_t configure_center_kernels();
 hipError_t configure_pa_kernels();
 hipError_t configure_mtrc_kernels();
+hipError_t configure_ifds_kernels();
 hipError_t configure_debug_kernels();
 
 namespace {
@@ -49,7 +50,7 @@ hipError_t configure_kernels() {
     static hipError_t result = hipSuccess;
     std::call_once(once, [] {
         hipError_t errs[] = {configure_ra_kernels(), configure_center_kernels(), configure_pa_kernels(),
-                             configure_mtrc_kernels(), configure_debug_kernels()};
+                             configure_mtrc_kernels(), configure_debug_kernels(), configure_ifds_kernels()};
         for (hipError_t e : errs)
             if (e != hipSuccess) result = e;
     });
@@ -477,6 +478,37 @@ ISAR_API int isar_run_frames(isar_context* ctx, int frames, const void* d_echo, 
     return ctx->run_frames(frames, d_echo, d_velocity, reinterpret_cast<const long long*>(d_frame_rows), d_phi0,
                            d_img, reinterpret_cast<const long long*>(d_frame_img), *opt,
                            static_cast<hipStream_t>(stream));
+}
+
+ISAR_API int isar_pulse_compress(const isar_radar_params* p, int echoes, int len, const int16_t* d_iq,
+                                 const double* d_velocity, void* d_out, int32_t* d_offsets, void* stream) {
+    if (!p || !d_iq || !d_velocity || !d_out || echoes < 1 || len < 1)
+        return isar::fail(ISAR_ERR_INVALID, "isar_pulse_compress: bad arguments");
+    const int nfft = isar::next_pow2_ref(len);
+    const int S = static_cast<int>(static_cast<double>(p->Fs) * p->Tp);
+    if (!isar::pulse_compress_supported(nfft))
+        return isar::fail(ISAR_ERR_INVALID, "isar_pulse_compress: echo length " + std::to_string(len) +
+                                          " gives an unsupported transform size " + std::to_string(nfft));
+    if (S < 2 || S > nfft) return isar::fail(ISAR_ERR_INVALID, "isar_pulse_compress: reference chirp (Fs Tp samples) out of range");
+    hipError_t ce = isar::configure_kernels();
+    if (ce != hipSuccess) return isar::fail(ISAR_ERR_DEVICE, std::string("kernel configuration: ") + hipGetErrorString(ce));
+    isar::PcJob P{};
+    P.iq = d_iq;
+    P.velocity = d_velocity;
+    P.out = static_cast<float2*>(d_out);
+    P.offsets = d_offsets;
+    P.echoes = echoes;
+    P.len = len;
+    P.S = S;
+    P.cut = isar::kRangeNumIfdsPc;
+    P.fc = static_cast<double>(p->fc);
+    P.band_width = static_cast<double>(p->band_width);
+    P.tp = p->Tp;
+    (void)hipGetLastError();
+    isar::launch_pulse_compress(P, nfft, static_cast<hipStream_t>(stream));
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return isar::fail(ISAR_ERR_DEVICE, std::string("isar_pulse_compress: ") + hipGetErrorString(e));
+    return ISAR_OK;
 }
 
 ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows) {

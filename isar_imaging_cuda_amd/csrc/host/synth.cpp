@@ -56,6 +56,13 @@ int isar_synth_preset(int id, isar_scene_config* c) {
     c->quantize = 1;
     c->seed = 0x1234ABCDULL + static_cast<uint64_t>(id);
     switch (id) {
+    case 6:  // IFDS raw echoes: 128 pulses of 6000 samples at 32 MHz, 20 MHz chirp of 40 us
+        c->echo_num = 128; c->range_num = 6000; c->n_scatterers = 40;
+        c->phase_error_model = ISAR_PHASE_UNIFORM;
+        c->box_x = 60; c->box_y = 60; c->box_z = 0;
+        c->band_width = 20e6; c->fs = 32e6; c->tp = 40e-6;
+        c->omega = 0.02; c->v = 20; c->a = 0; c->walk_cells = 3; c->snr_db = 20;
+        return 0;
     case 1:  // 128 x 1024, 40 scatterers in 20 x 20 m, v = 20 m/s untracked, iid phase, 20 dB
         c->echo_num = 128; c->range_num = 1024; c->n_scatterers = 40;
         c->box_x = 20; c->box_y = 20; c->box_z = 0;
@@ -222,6 +229,95 @@ int isar_synth_frame(const isar_scene_config* c, float* echo, double* velocity, 
     }
     if (truth_phase) std::copy(perr.begin(), perr.end(), truth_phase);
     if (truth_walk) std::copy(walk.begin(), walk.end(), truth_walk);
+    return 0;
+}
+
+int isar_synth_ifds(const isar_scene_config* c, int len, float* iq, double* velocity) {
+    if (!c || !iq || !velocity || c->echo_num < 1 || len < 2 || c->n_scatterers < 0) return -1;
+    const int M = c->echo_num, Sc = c->n_scatterers;
+    const int S = static_cast<int>(c->fs * c->tp);   // reference chirp samples (common.cu:671)
+    if (S < 2 || S > len) return -1;
+    const double T = M / c->prf;
+    const double cell = kC / (2.0 * c->fs);           // metres per sample of two-way delay
+    std::vector<double> sx(Sc), sy(Sc), sz(Sc);
+    std::vector<cd> amp(Sc);
+    {
+        Rng r(stream_seed(c->seed, kScene, 0));
+        for (int i = 0; i < Sc; ++i) {
+            sx[i] = (r.uniform() - 0.5) * c->box_x;
+            sy[i] = (r.uniform() - 0.5) * c->box_y;
+            sz[i] = (r.uniform() - 0.5) * c->box_z;
+            double g1 = r.gauss(), g2 = r.gauss();
+            amp[i] = cd(g1, g2);
+        }
+    }
+    std::vector<double> perr(M, 0.0);
+    if (c->phase_error_model == ISAR_PHASE_UNIFORM) {
+        Rng r(stream_seed(c->seed, kPhase, 0));
+        for (int m = 0; m < M; ++m) perr[m] = (2.0 * r.uniform() - 1.0) * kPi;
+    }
+    const double cel = std::cos(c->elevation), sel = std::sin(c->elevation);
+    const int d0 = (len - S) / 2;
+    const double c2 = kPi * c->band_width / c->tp;
+    std::vector<double> pw(M, 0.0);
+#pragma omp parallel
+    {
+        std::vector<cd> acc(len), u(S);
+#pragma omp for schedule(dynamic, 2)
+        for (int m = 0; m < M; ++m) {
+            const double t = m / c->prf;
+            const double vm = c->v + c->a * t;
+            const double v2 = 2.0 * vm / kC, v2_ = 1.0 - v2;
+            const double c1 = -2.0 * kPi * c->fc * v2 / v2_;
+            for (int k = 0; k < S; ++k) {   // the transmitted chirp as the receiver models it
+                const double tk = v2_ * (-c->tp / 2.0 + k * c->tp / (static_cast<double>(S) - 1.0));
+                const double ph = (c1 + c2 * tk) * tk;
+                u[k] = cd(std::cos(ph), std::sin(ph));
+            }
+            const double walk = c->walk_cells * cell * (2.0 * t / (T > 0 ? T : 1.0) - 1.0);
+            const double th = c->omega * (t - T / 2);
+            const double ct = std::cos(th), st = std::sin(th);
+            std::fill(acc.begin(), acc.end(), cd(0, 0));
+            for (int i = 0; i < Sc; ++i) {
+                const double dR = walk + (sy[i] * ct - sx[i] * st) * cel + sz[i] * sel;
+                const int d = d0 + static_cast<int>(std::lround(dR / cell));
+                const double phi = -(4 * kPi / kC) * c->fc * dR + perr[m];
+                const cd z = amp[i] * cd(std::cos(phi), std::sin(phi));
+                for (int k = std::max(0, -d); k < S && d + k < len; ++k) acc[d + k] += z * u[k];
+            }
+            double p = 0;
+            for (int k = 0; k < len; ++k) {
+                p += std::norm(acc[k]);
+                iq[2 * (static_cast<size_t>(m) * len + k)] = static_cast<float>(acc[k].real());
+                iq[2 * (static_cast<size_t>(m) * len + k) + 1] = static_cast<float>(acc[k].imag());
+            }
+            pw[m] = p;
+            velocity[m] = vm;
+        }
+    }
+    double psig = 0;
+    for (double p : pw) psig += p;
+    psig /= static_cast<double>(M) * len;
+    const bool noisy = c->snr_db < 200 && psig > 0;
+    const double nsig = noisy ? std::sqrt(psig / std::pow(10.0, c->snr_db / 10.0) / 2.0) : 0.0;
+    const size_t tot = 2 * static_cast<size_t>(M) * len;
+    if (noisy) {
+#pragma omp parallel for schedule(static)
+        for (int m = 0; m < M; ++m) {
+            Rng r(stream_seed(c->seed, kNoise, static_cast<uint64_t>(m)));
+            for (int q = 0; q < 2 * len; ++q) iq[2 * static_cast<size_t>(m) * len + q] += static_cast<float>(nsig * r.gauss());
+        }
+    }
+    if (c->peak_adc > 0) {
+        float pk = 0;
+        for (size_t i = 0; i < tot; ++i) pk = std::max(pk, std::fabs(iq[i]));
+        const double g = pk > 0 ? c->peak_adc / pk : 1.0;
+        for (size_t i = 0; i < tot; ++i) {
+            double v = iq[i] * g;
+            if (c->quantize) v = std::max(-32768.0, std::min(32767.0, std::nearbyint(v)));
+            iq[i] = static_cast<float>(v);
+        }
+    }
     return 0;
 }
 

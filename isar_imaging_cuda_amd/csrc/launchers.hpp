@@ -39,6 +39,10 @@ void launch_col_window(const Job& J, hipStream_t s);
 void launch_mtrc_image(const Job& J, hipStream_t s);
 void launch_post_nomtrc(const Job& J, hipStream_t s);
 
+// IFDS pulse compression (common.cu:583-604, 635-746)
+bool pulse_compress_supported(int nfft);
+void launch_pulse_compress(const PcJob& P, int nfft, hipStream_t s);
+
 // learned warm start: BF16 matrix-core GEMMs with FP32 accumulation and their helpers
 void launch_gemm_f32_bf16w(int m, int n, int k, const float* a, long sam, long sak, const uint16_t* w, long sbk,
                            long sbn, float* c, long scm, const float* bias, int relu, hipStream_t s);

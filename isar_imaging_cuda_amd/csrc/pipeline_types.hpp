@@ -14,6 +14,7 @@
 namespace isar {
 
 constexpr int kRangeCut = 512;        // RANGE_NUM_CUT (common.cuh:71)
+constexpr int kRangeNumIfdsPc = 4096; // RANGE_NUM_IFDS_PC (common.cuh:72)
 constexpr int kRaChunkRowsDefault = 8;   // rows per workgroup in the alignment update kernel
 constexpr int kCutBandRows = 16;      // rows per workgroup when cutting the range window
 
@@ -81,6 +82,16 @@ struct Job {
     const float* win_range;       // N: Hamming window over range cells (genHammingVec, common.cu:260-266)
     // output
     float* img;                   // K x M x W
+};
+
+// One batch of IFDS echoes to pulse-compress (common.cu:583-604, 635-746).
+struct PcJob {
+    const int16_t* iq;            // echoes x len I/Q pairs
+    const double* velocity;       // echoes
+    float2* out;                  // echoes x cut
+    int32_t* offsets;             // echoes (cut offsets), or null
+    int echoes, len, S, cut;      // S: reference chirp samples (int)(Fs Tp)
+    double fc, band_width, tp;
 };
 
 // First echo row of frame f and the start of its image.

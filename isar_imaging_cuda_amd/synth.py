@@ -75,11 +75,27 @@ def radar_header(cfg: SceneConfig) -> dict:
     return {"band_width": B.value, "fc": fc.value, "Tp": tp.value, "Fs": fs.value}
 
 
+def ifds(cfg: SceneConfig):
+    """Raw IFDS echoes: (iq complex64 (M, cfg.range_num), velocity float64 (M,))."""
+    M, L = cfg.echo_num, cfg.range_num
+    iq = np.zeros((M, L), np.complex64)
+    vel = np.zeros(M, np.float64)
+    L_ = lib()
+    L_.isar_synth_ifds.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    if L_.isar_synth_ifds(C.byref(cfg), L, iq.ctypes.data_as(C.c_void_p), vel.ctypes.data_as(C.c_void_p)) != 0:
+        raise ValueError("isar_synth_ifds failed")
+    return iq, vel
+
+
 def write_wbd(path: str | Path, echo: np.ndarray, cfg: SceneConfig, range_m=None, velocity=None,
-              azimuth=None, pitch=None) -> None:
+              azimuth=None, pitch=None, ifds: bool = False) -> None:
+    """Frames of 256-byte header + int16 I/Q. IFDS files carry twice the sampling rate in the
+    header, which the reader halves (common.cu:823)."""
     e = np.ascontiguousarray(echo, dtype=np.complex64)
     K, N = e.shape
-    h = radar_header(cfg)
+    h = dict(radar_header(cfg))
+    if ifds:
+        h["Fs"] = 2 * h["Fs"]
 
     def arr(a):
         return None if a is None else np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(C.c_void_p)

@@ -604,6 +604,46 @@ void orc_mtrc(double* x_, const orc_params* p, int data_type) {
     if (data_type == 1) flip_rows(X, M, W);   // :72-75 fliplr
 }
 
+// IFDS pulse compression (common.cu:583-604 multiThreadIFDS, 635-746 pulseCompression): the
+// reference chirp with the echo's velocity, FFT matched filter, and the cut around the peak
+// (cutRangeProfile, common.cu:494-518, with cols = NFFT).
+int orc_pulse_compress(const float* iq, int len, double velocity, const orc_params* p, int cut, double* out) {
+    const int nfft = orc_next_pow2(len);
+    const int S = static_cast<int>(static_cast<double>(p->Fs) * p->Tp);
+    if (len < 1 || cut < 1 || cut > nfft || S < 2 || S > nfft) return -1;
+    const double c = 3e8;   // LIGHT_SPEED (common.cuh:69)
+    const double v2 = 2.0 * velocity / c;
+    const double v2_ = 1.0 - v2;
+    const double c1 = -2.0 * M_PI * static_cast<double>(p->fc) * v2 / v2_;
+    const double c2 = M_PI * static_cast<double>(p->band_width) / p->Tp;
+    std::vector<double> ham = hamming(S);
+    std::vector<cd> ref(nfft, cd(0.0, 0.0)), dat(nfft, cd(0.0, 0.0));
+    for (int k = 0; k < S; ++k) {
+        const double t = v2_ * (-p->Tp / 2.0 + k * p->Tp / (static_cast<double>(S) - 1.0));   // genTkPulseCompression
+        const double ph = (c1 + c2 * t) * t;                                                  // genRefPulseCompression
+        ref[k] = ham[k] * cd(std::cos(ph), std::sin(ph));
+    }
+    for (int k = 0; k < len; ++k) dat[k] = cd(iq[2 * k], iq[2 * k + 1]);
+    fft_any(ref.data(), nfft, -1);
+    fft_any(dat.data(), nfft, -1);
+    for (int k = 0; k < nfft; ++k) dat[k] = std::conj(ref[k]) * dat[k];   // elementwiseMultiplyConjA
+    fft_any(dat.data(), nfft, +1);
+    int imax = 0;
+    double best = -1.0;
+    for (int k = 0; k < nfft; ++k) {
+        dat[k] /= static_cast<double>(nfft);
+        const double a = std::abs(dat[k].real()) + std::abs(dat[k].imag());   // Icamax: first max of |Re|+|Im|
+        if (a > best) { best = a; imax = k; }
+    }
+    int off = std::max(imax - cut / 2, 0);
+    off = std::min(off, nfft - cut);
+    for (int j = 0; j < cut; ++j) {
+        out[2 * j] = dat[off + j].real();
+        out[2 * j + 1] = dat[off + j].imag();
+    }
+    return off;
+}
+
 double orc_image_entropy(const float* img, int n) {
     double tot = 0;
     for (int i = 0; i < n; ++i) tot += static_cast<double>(img[i]) * img[i];

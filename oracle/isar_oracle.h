@@ -5,9 +5,9 @@
  * against, and the CPU baseline that bench.py times. Product code never links, loads or calls it.
  *
  * Every stage follows the reference source literally (same stage order, same data flow, same
- * integer decisions); the only deliberate differences are listed in oracle/README.md and are
- * resolutions of undefined behaviour in the reference (uninitialised buffers, out-of-bounds
- * reads, NaN-producing divisions) to their intended semantics.
+ * integer decisions); the only deliberate differences are resolutions of undefined behaviour in
+ * the reference (uninitialised buffers, out-of-bounds reads, NaN-producing divisions) to their
+ * intended semantics, each marked with its quirk label (Q2 ... Q13) where it is resolved.
  *
  * Reference files (paths relative to the reference repository):
  *   isar_imaging/isar_main.cu:3-227            pipeline order            -> orc_isar_main
@@ -18,6 +18,7 @@
  *   isar_imaging/phase_adjustment.cu:4-41     dopplerTracking            -> orc_doppler_tracking
  *   isar_imaging/phase_adjustment.cu:167-346  fastEntropy                -> orc_fast_entropy
  *   isar_imaging/mtrc.cu:4-192                MTRC (chirp-z)             -> orc_mtrc, orc_czt
+ *   isar_imaging/common.cu:583-746            IFDS pulse compression     -> orc_pulse_compress
  */
 #ifndef ISAR_ORACLE_H_
 #define ISAR_ORACLE_H_
@@ -137,7 +138,15 @@ void orc_mtrc(double* x, const orc_params* p, int data_type);
 
 /* Synthetic-frame-free helpers used by the tests. */
 int orc_next_pow2(int n);                            /* common.cu:521-529 */
-double orc_image_entropy(const float* img, int n);   /* -sum p ln p, p = img^2/sum img^2 */
+double orc_image_entropy(const float* img, int n);
+
+/* IFDS pulse compression of one echo (common.cu:583-604, 635-746): iq = `len` complex samples
+ * (interleaved), NFFT = orc_next_pow2(len); the reference chirp of S = (int)(Fs Tp) samples,
+ * t_k = (1 - 2v/c)(-Tp/2 + k Tp/(S-1)), ref_k = hamming_S[k] exp(j (c1 + c2 t_k) t_k) with
+ * c1 = -2 pi fc v2 / (1 - v2), c2 = pi B / Tp, zero-padded to NFFT; out = IFFT(conj(FFT(ref))
+ * FFT(iq)) / NFFT cut to `cut` samples from offset = clamp(i* - cut/2, 0, NFFT - cut), i* the
+ * first index of max |Re|+|Im| (cublasIcamax). Returns the offset, or -1 for bad sizes. */
+int orc_pulse_compress(const float* iq, int len, double velocity, const orc_params* p, int cut, double* out);   /* -sum p ln p, p = img^2/sum img^2 */
 
 #ifdef __cplusplus
 }

@@ -253,6 +253,19 @@ def next_pow2(n: int) -> int:
     return lib().orc_next_pow2(n)
 
 
+def pulse_compress(iq: np.ndarray, velocity: float, p: OrcParams, cut: int = 4096) -> tuple[np.ndarray, int]:
+    """IFDS pulse compression of one echo (complex64 samples) -> (cut complex128 samples, offset)."""
+    a = np.ascontiguousarray(iq, dtype=np.complex64)
+    out = np.zeros(cut, np.complex128)
+    L = lib()
+    L.orc_pulse_compress.restype = C.c_int
+    L.orc_pulse_compress.argtypes = [C.c_void_p, C.c_int, C.c_double, C.POINTER(OrcParams), C.c_int, C.c_void_p]
+    off = L.orc_pulse_compress(_ptr(a), a.size, float(velocity), C.byref(p), cut, _ptr(out))
+    if off < 0:
+        raise ValueError("pulse compression: unsupported sizes")
+    return out, off
+
+
 def image_entropy(img: np.ndarray) -> float:
     a = np.ascontiguousarray(img, dtype=np.float32)
     return lib().orc_image_entropy(_ptr(a), a.size)

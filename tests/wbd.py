@@ -1,4 +1,4 @@
-"""Synthetic STRETCH .wbd dwells for the drop-in ingest tests, and an independent reader.
+"""Synthetic STRETCH and IFDS .wbd dwells for the drop-in ingest tests, and an independent reader.
 
 Layout (common.cu:796-899, 1082-1100): frames of frame_len bytes = 256-byte header + int16 I/Q
 pairs. Words (uint32, little endian) of the first header: [4] frame_len/4, [12] fc MHz, [13] B MHz,
@@ -26,6 +26,20 @@ def make_dwell(directory: Path, pulses: int = 256, range_num: int = 1024, seed: 
     rng_m = 10000.0 + 20.0 * np.arange(pulses) / cfg.prf
     synth.write_wbd(path, echo, cfg, range_m=rng_m, velocity=vel, azimuth=az, pitch=pitch)
     return path, echo, vel, cfg
+
+
+def make_ifds_dwell(directory: Path, pulses: int = 160, seed: int = 91, polar: str = "1101"):
+    """Writes one IFDS file (raw chirp echoes, preset 6); returns (path, iq complex64 (P, len),
+    velocity as stored, config)."""
+    cfg = synth.preset(6, echo_num=pulses, seed=seed)
+    iq, vel = synth.ifds(cfg)
+    directory.mkdir(parents=True, exist_ok=True)
+    path = directory / f"test_dwell_01_{polar}.wbd"
+    az = 30.0 + 0.002 * np.arange(pulses)
+    pitch = np.full(pulses, 5.0)
+    rng_m = 10000.0 + 20.0 * np.arange(pulses) / cfg.prf
+    synth.write_wbd(path, iq, cfg, range_m=rng_m, velocity=vel, azimuth=az, pitch=pitch, ifds=True)
+    return path, iq, vel, cfg
 
 
 def read_wbd(path: Path):
