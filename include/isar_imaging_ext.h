@@ -121,6 +121,20 @@ ISAR_API int isar_run_frames(isar_context* ctx, int frames, const void* d_echo, 
 ISAR_API int isar_pulse_compress(const isar_radar_params* p, int echoes, int len, const int16_t* d_iq,
                                  const double* d_velocity, void* d_out, int32_t* d_offsets, void* stream);
 
+/* High-resolution range profiles (getHRRP, common.cu:541-580, called at isar_main.cu:57-67; the
+ * reference keeps them in the DLL's device buffer d_hrrp, isar_imaging_export.cpp:18, 176-178).
+ * After this call, runs with if_hrrp = 1 also write each frame's M x N profiles as complex
+ * float, frame f at d_hrrp + f M N:
+ *   STRETCH: ifftshift(FFT(hamming_N * row)) of the rows after high-speed compensation;
+ *   IFDS: the pulse-compressed rows themselves, which getHRRP's shift-transform-shift returns
+ *         unchanged.
+ * The buffer holds max_frames * M * N complex floats. NULL stops the output. */
+ISAR_API int isar_set_hrrp_output(isar_context* ctx, void* d_hrrp);
+
+/* Drop-in session: after isarMainSingle with if_hrrp (and imagingMemInit with if_hrrp), copies
+ * the window's M x N profiles (complex float) to h_hrrp. Synchronous. */
+ISAR_API int isar_session_hrrp(float* h_hrrp);
+
 /* Rows of the range-alignment update handled per workgroup (2, 4, 8 or 16; default 8). The
  * choice changes the summation order of the group energies, not the algorithm. */
 ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows);
@@ -152,8 +166,8 @@ enum isar_stage {
     ISAR_STAGE_IFDS_PRE = 0, ISAR_STAGE_RA_PAIR0 = 1, ISAR_STAGE_RA_CORR = 2, ISAR_STAGE_RA_UPD = 3,
     ISAR_STAGE_CENTER = 4, ISAR_STAGE_CUT = 5, ISAR_STAGE_PA_COLFFT = 6, ISAR_STAGE_PA_SELECT = 7,
     ISAR_STAGE_AUTOFOCUS = 8, ISAR_STAGE_MTRC_ROWS = 9, ISAR_STAGE_MTRC_CZT = 10, ISAR_STAGE_COL_WINDOW = 11,
-    ISAR_STAGE_MTRC_IMAGE = 12, ISAR_STAGE_POST_NOMTRC = 13,
-    ISAR_STAGE_COUNT = 14
+    ISAR_STAGE_MTRC_IMAGE = 12, ISAR_STAGE_POST_NOMTRC = 13, ISAR_STAGE_HRRP = 14,
+    ISAR_STAGE_COUNT = 15
 };
 
 /* Record a HIP event pair around every launch of kernel family `stage` (on the run stream) in

@@ -50,7 +50,7 @@ class _Trace(C.Structure):
 # kernel families (enum isar_stage in include/isar_imaging_ext.h)
 STAGES = {"ifds_pre": 0, "ra_pair0": 1, "ra_corr": 2, "ra_upd": 3, "center": 4, "cut": 5, "pa_colfft": 6,
           "pa_select": 7, "autofocus": 8, "mtrc_rows": 9, "mtrc_czt": 10, "col_window": 11, "mtrc_image": 12,
-          "post_nomtrc": 13}
+          "post_nomtrc": 13, "hrrp": 14}
 
 _lib = None
 
@@ -229,6 +229,13 @@ class Imager:
         device buffers; asynchronous on `stream`."""
         _check(lib().isar_get_phases(self._h, frames, C.c_void_p(start_ptr or 0), C.c_void_p(final_ptr or 0),
                                      C.c_void_p(stream or 0)), "isar_get_phases")
+
+    def set_hrrp_output(self, hrrp_ptr: int | None) -> None:
+        """Device buffer (max_frames x M x N complex64) for the range profiles that runs with
+        if_hrrp write; None stops the output."""
+        L = lib()
+        L.isar_set_hrrp_output.argtypes = [C.c_void_p, C.c_void_p]
+        _check(L.isar_set_hrrp_output(self._h, C.c_void_p(hrrp_ptr or 0)), "isar_set_hrrp_output")
 
     def set_ra_chunk(self, rows: int) -> None:
         """Rows per workgroup in the range-alignment update (2, 4, 8 or 16)."""

@@ -3,8 +3,9 @@
 // isarMainSingle } x n -> imagingMemDest. Built against include/isar_imaging_export.h only.
 //
 // usage: isar_calling_app <dir> <window_len> <imaging_stride> <n_images> <out_prefix> [polar] [mtrc] [hpc] [data_type]
-//                         [polar_type=1] [if_mtrc=1] [if_hpc=1]
-// writes <out_prefix>_<i>.bin (raw float32 image, window_len x 512) and <out_prefix>_0.txt.
+//                         [hrrp]  (defaults: polar_type 1, if_mtrc 1, if_hpc 1, data_type 2, if_hrrp 0)
+// writes <out_prefix>_<i>.bin (raw float32 image, window_len x 512) and <out_prefix>_0.txt; with
+// if_hrrp also <out_prefix>_hrrp_<i>.bin (window_len x range_num complex float range profiles).
 // n_images = 0 only parses the directory (no GPU call) and prints the decoded headers.
 #include <chrono>
 #include <cstdio>
@@ -17,7 +18,7 @@
 
 int main(int argc, char** argv) {
     if (argc < 6) {
-        std::fprintf(stderr, "usage: %s dir window_len imaging_stride n_images out_prefix [polar] [mtrc] [hpc] [data_type]\n",
+        std::fprintf(stderr, "usage: %s dir window_len imaging_stride n_images out_prefix [polar] [mtrc] [hpc] [data_type] [hrrp]\n",
                      argv[0]);
         return 2;
     }
@@ -33,7 +34,7 @@ int main(int argc, char** argv) {
     const int sampling_stride = 1;
     int window_head = 0;
     const int option_alignment = 0, option_phase = 1;
-    const bool if_hrrp = false;
+    const bool if_hrrp = argc > 10 ? std::atoi(argv[10]) != 0 : false;
 
     vec1D_DBL dataN;
     vec1D_FLT turnAngle;
@@ -81,6 +82,15 @@ int main(int argc, char** argv) {
         std::ofstream out(prefix + "_" + std::to_string(i) + ".bin", std::ios::binary);
         out.write(reinterpret_cast<const char*>(img.data()), static_cast<std::streamsize>(img.size() * sizeof(float)));
         if (i == 0) writeFileFLT(prefix + "_0.txt", img.data(), img.size());
+        if (if_hrrp) {
+            vec1D_FLT hrrp(img.size() / 512 * static_cast<size_t>(data_type == 1 ? 4096 : (frame_len - 256) / 4) * 2);
+            if (isar_session_hrrp(hrrp.data()) != ISAR_OK) {
+                std::fprintf(stderr, "isar_session_hrrp: %s\n", isar_last_error());
+                return 7;
+            }
+            std::ofstream h(prefix + "_hrrp_" + std::to_string(i) + ".bin", std::ios::binary);
+            h.write(reinterpret_cast<const char*>(hrrp.data()), static_cast<std::streamsize>(hrrp.size() * sizeof(float)));
+        }
         window_head += imaging_stride;
     }
     imagingMemDest(data_type, if_hpc, if_hrrp);

@@ -43,6 +43,7 @@ struct Session {
     float2* d_proc = nullptr;   // window handed to the pipeline (reference d_data_proc)
     double* d_velocity = nullptr;
     float* d_img = nullptr;
+    float2* d_hrrp = nullptr;       // M x N range profiles when imagingMemInit had if_hrrp
     std::vector<double> h_velocity;
     // IFDS: raw echoes staged for pulse compression (common.cu:583-604)
     int16_t* d_iq = nullptr;
@@ -164,12 +165,14 @@ void free_buffers() {
     if (g.ctx) isar_destroy(g.ctx);
     g.ctx = nullptr;
     for (void* p : {static_cast<void*>(g.d_pp[0]), static_cast<void*>(g.d_pp[1]), static_cast<void*>(g.d_proc),
-                    static_cast<void*>(g.d_velocity), static_cast<void*>(g.d_img), static_cast<void*>(g.d_iq),
+                    static_cast<void*>(g.d_velocity), static_cast<void*>(g.d_img), static_cast<void*>(g.d_hrrp),
+                    static_cast<void*>(g.d_iq),
                     static_cast<void*>(g.d_pc_velocity)})
         if (p) (void)hipFree(p);
     g.d_pp[0] = g.d_pp[1] = g.d_proc = nullptr;
     g.d_velocity = nullptr;
     g.d_img = nullptr;
+    g.d_hrrp = nullptr;
     g.d_iq = nullptr;
     g.d_pc_velocity = nullptr;
     g.iq_len = 0;
@@ -217,7 +220,6 @@ DLL_EXPORT_API void imagingMemInit(vec1D_FLT* img, vec1D_INT* dataWFileSn, vec1D
                                    vec1D_COM_FLT* dataW, const int& window_len, const int& frame_len, const int& data_type,
                                    const bool& if_hpc, const bool& if_hrrp) {
     (void)if_hpc;
-    (void)if_hrrp;
     free_buffers();
     // size rules and validation (isar_imaging_export.cpp:129-155)
     isar_radar_params p = g.paras;
@@ -238,6 +240,9 @@ DLL_EXPORT_API void imagingMemInit(vec1D_FLT* img, vec1D_INT* dataWFileSn, vec1D
               hipMalloc(&g.d_proc, M * N * sizeof(float2)) == hipSuccess &&
               hipMalloc(&g.d_velocity, M * sizeof(double)) == hipSuccess &&
               hipMalloc(&g.d_img, M * W * sizeof(float)) == hipSuccess;
+    // isar_imaging_export.cpp:176-178: the profile buffer exists only when if_hrrp
+    if (ok && if_hrrp)
+        ok = hipMalloc(&g.d_hrrp, M * N * sizeof(float2)) == hipSuccess && isar_set_hrrp_output(g.ctx, g.d_hrrp) == ISAR_OK;
     if (ok && data_type == kIfds) {
         g.iq_len = (frame_len - 256) / 4;
         ok = hipMalloc(&g.d_iq, M * static_cast<size_t>(g.iq_len) * 2 * sizeof(int16_t)) == hipSuccess &&
@@ -327,6 +332,15 @@ DLL_EXPORT_API int dataExtracting(vec1D_INT* dataWFileSn, vec1D_DBL* dataNOut, v
     return EXIT_SUCCESS;
 }
 
+ISAR_API int isar_session_hrrp(float* h_hrrp) {
+    if (!g.ctx || !g.d_hrrp) return err("no range-profile buffer: imagingMemInit without if_hrrp"), ISAR_ERR_STATE;
+    if (!h_hrrp) return err("null output pointer"), ISAR_ERR_INVALID;
+    const size_t bytes = sizeof(float2) * static_cast<size_t>(g.paras.echo_num) * g.paras.range_num;
+    const hipError_t e = hipMemcpy(h_hrrp, g.d_hrrp, bytes, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return err(std::string("isar_session_hrrp: ") + hipGetErrorString(e)), ISAR_ERR_DEVICE;
+    return ISAR_OK;
+}
+
 DLL_EXPORT_API void isarMainSingle(float* h_img, const int& data_type, const int& option_alignment, const int& option_phase,
                                    const bool& if_hrrp, const bool& if_hpc, const bool& if_mtrc) {
     if (!g.ctx || !h_img) { err("isarMainSingle before imagingMemInit"); return; }
@@ -354,7 +368,6 @@ DLL_EXPORT_API void writeFileFLT(const std::string& outFilePath, const float* da
 DLL_EXPORT_API void imagingMemDest(const int& data_type, const bool& if_hpc, const bool& if_hrrp) {
     (void)data_type;
     (void)if_hpc;
-    (void)if_hrrp;
     free_buffers();
 }
 

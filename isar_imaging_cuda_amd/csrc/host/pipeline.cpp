@@ -259,6 +259,7 @@ int Context::run_frames(int frames, const void* echo, const double* velocity, co
     J.ra_chunk = ra_chunk;
     J.phi0_in = static_cast<const float2*>(phi0);
     J.img = img;
+    J.hrrp = opt.if_hrrp && mode != ISAR_RUN_BACK ? hrrp_out : nullptr;
     const int M = J.M;
 
     (void)hipGetLastError();
@@ -266,6 +267,7 @@ int Context::run_frames(int frames, const void* echo, const double* velocity, co
     if (probe_stage >= 0 && probe_used + 16 > static_cast<int>(probe_events.size()) / 2) probe_collect();
     if (mode != ISAR_RUN_BACK) {
         if (J.data_type == 1) probed(ISAR_STAGE_IFDS_PRE, s, [&] { launch_ifds_pre(J, s); });
+        else if (J.hrrp) probed(ISAR_STAGE_HRRP, s, [&] { launch_hrrp(J, s); });
         probed(ISAR_STAGE_RA_PAIR0, s, [&] { launch_ra_pair0(J, s); });
         int pair_base = M / 2;
         for (int st = 2; st < M; st *= 2) {
@@ -520,6 +522,12 @@ ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows) {
 }
 
 ISAR_API int isar_get_ra_chunk(isar_context* ctx) { return ctx ? ctx->ra_chunk : 0; }
+
+ISAR_API int isar_set_hrrp_output(isar_context* ctx, void* d_hrrp) {
+    if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");
+    ctx->hrrp_out = static_cast<float2*>(d_hrrp);
+    return ISAR_OK;
+}
 
 ISAR_API int isar_autotune(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity, float* d_img,
                            const isar_options* opt, int repeats, int32_t* chosen, void* stream) {

@@ -53,6 +53,7 @@ struct Context {
     int run_frames(int frames, const void* echo, const double* velocity, const long long* frame_row,
                    const void* phi0, float* img, const long long* frame_img, const isar_options& opt,
                    hipStream_t stream);
+    float2* hrrp_out = nullptr;           // max_frames x M x N profiles written by if_hrrp runs, or null
     int ra_chunk = kRaChunkRowsDefault;   // tunable: rows per workgroup in the alignment update
     // times every range-alignment chunk size on these frames and keeps the fastest (synchronous)
     int autotune(int frames, const void* echo, const double* velocity, const long long* frame_row, float* img,

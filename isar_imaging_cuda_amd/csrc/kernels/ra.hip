@@ -25,8 +25,10 @@ __device__ __forceinline__ int out_pos(int tid, int e) {
 }
 
 // Spectrum of row m of frame f with HPC (FP64 phase), window and (-1)^n centring applied, in
-// the first-pass layout (range_alignment.cu:13-18, high_speed_compensation.cu:4-54).
-template <int N>
+// the first-pass layout (range_alignment.cu:13-18, high_speed_compensation.cu:4-54). With
+// if_hrrp the STRETCH rows carry the window twice (getHRRP windows them in place first,
+// common.cu:563-566) unless ONE_WINDOW asks for the rows as getHRRP transforms them.
+template <int N, bool ONE_WINDOW = false>
 __device__ __forceinline__ void load_row_raw(float2 (&x)[kE], const Job& J, int f, int m, int tid_) {
     const int tid = opaque(tid_);
     const long long erow = echo_first_row(J, f) + m;
@@ -34,7 +36,7 @@ __device__ __forceinline__ void load_row_raw(float2 (&x)[kE], const Job& J, int 
                                          : J.echo + static_cast<size_t>(erow) * N;
     const bool hpc = J.if_hpc && J.data_type == 2;
     const double vk = hpc ? J.hpc_k * J.velocity[erow] : 0.0;
-    const bool hrrp_window = J.if_hrrp && J.data_type == 2;
+    const bool hrrp_window = !ONE_WINDOW && J.if_hrrp && J.data_type == 2;
 #pragma unroll
     for (int e = 0; e < kE; ++e) {
         const int n = in_pos<N>(tid, e);
@@ -171,10 +173,30 @@ __global__ __launch_bounds__(N / kE) void k_ifds_pre(Job J) {
     float2 x[kE];
 #pragma unroll
     for (int e = 0; e < kE; ++e) x[e] = src[(in_pos<N>(tid, e) + N / 2) & (N - 1)];
+    // getHRRP for IFDS (common.cu:546-558) undoes this stage, ifftshift(FFT(ifftshift(pre))) =
+    // the input row, so the profile is the row itself
+    if (J.hrrp) {
+#pragma unroll
+        for (int e = 0; e < kE; ++e) J.hrrp[off + ((in_pos<N>(tid, e) + N / 2) & (N - 1))] = x[e];
+    }
     fft<N, kE, +1>(x, lds, tid);
     const float inv = 1.0f / N;
 #pragma unroll
     for (int e = 0; e < kE; ++e) J.pre[off + ((out_pos<N>(tid, e) + N / 2) & (N - 1))] = cscale(x[e], inv);
+}
+
+// getHRRP for STRETCH (common.cu:561-571): ifftshift(FFT(hamming_N * row)) of the compensated
+// rows, one workgroup per row. The ifftshift is the (-1)^n modulation load_row_raw applies.
+template <int N>
+__global__ __launch_bounds__(N / kE) void k_hrrp(Job J) {
+    HIP_DYNAMIC_SHARED(float2, lds);
+    const int m = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    float2 x[kE];
+    load_row_raw<N, true>(x, J, f, m, tid);
+    fft<N, kE, -1>(x, lds, tid);
+    float2* dst = J.hrrp + (static_cast<size_t>(f) * J.M + m) * N;
+#pragma unroll
+    for (int e = 0; e < kE; ++e) dst[out_pos<N>(tid, e)] = x[e];
 }
 
 // Merge level 0 fused with the spectrum preparation (range_alignment.cu:13-18 and :34-73 at
@@ -398,6 +420,7 @@ hipError_t configure_ra_kernels() {
     };
 #define ISAR_SET_RA(NN)                                                        \
     set(reinterpret_cast<const void*>(&k_ifds_pre<NN>), ra_lds<NN>());         \
+    set(reinterpret_cast<const void*>(&k_hrrp<NN>), ra_lds<NN>());             \
     set(reinterpret_cast<const void*>(&k_ra_pair0<NN>), ra_lds<NN>());         \
     set(reinterpret_cast<const void*>(&k_ra_corr<NN>), ra_lds<NN>());          \
     set(reinterpret_cast<const void*>(&k_ra_upd<NN>), ra_lds<NN>());
@@ -409,6 +432,10 @@ hipError_t configure_ra_kernels() {
 
 void launch_ifds_pre(const Job& J, hipStream_t s) {
     ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH(k_ifds_pre<NN>, dim3(J.M, J.K), dim3(NN / kE), ra_lds<NN>(), s, J));
+}
+
+void launch_hrrp(const Job& J, hipStream_t s) {
+    ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH(k_hrrp<NN>, dim3(J.M, J.K), dim3(NN / kE), ra_lds<NN>(), s, J));
 }
 void launch_ra_pair0(const Job& J, hipStream_t s) {
     ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH(k_ra_pair0<NN>, dim3(J.M / 2, J.K), dim3(NN / kE), ra_lds<NN>(), s, J));

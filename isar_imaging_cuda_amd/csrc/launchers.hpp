@@ -16,6 +16,7 @@ void report_config_error(const char* file, size_t lds_bytes);
 
 // range alignment (range_alignment.cu:4-78)
 void launch_ifds_pre(const Job& J, hipStream_t s);
+void launch_hrrp(const Job& J, hipStream_t s);
 void launch_ra_pair0(const Job& J, hipStream_t s);
 void launch_ra_corr(const Job& J, int stride, int pair_base, int nblocks, hipStream_t s);
 void launch_ra_upd(const Job& J, int stride, int pair_base, int nblocks, hipStream_t s);

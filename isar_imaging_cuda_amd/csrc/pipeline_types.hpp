@@ -80,8 +80,9 @@ struct Job {
     const float2* czt_post;       // W x M: exp(-j pi a_n k^2 / M) / F, the output chirp
     float2* delta;                // K x W: transformed column sums for the slow-time window quirk
     const float* win_range;       // N: Hamming window over range cells (genHammingVec, common.cu:260-266)
-    // output
+    // outputs
     float* img;                   // K x M x W
+    float2* hrrp;                 // K x M x N high-resolution range profiles (getHRRP), or null
 };
 
 // One batch of IFDS echoes to pulse-compress (common.cu:583-604, 635-746).

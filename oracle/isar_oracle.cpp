@@ -656,6 +656,48 @@ double orc_image_entropy(const float* img, int n) {
     return h;
 }
 
+namespace {
+// The rows as range alignment receives them, before getHRRP: the IFDS pre-stage
+// (isar_main.cu:15-27) or, for STRETCH, the HPC (:31-48).
+void front_rows(std::vector<cd>& S, const float* echo, const double* velocity, const orc_params* p,
+                const orc_options* opt) {
+    const int M = p->echo_num, N = p->range_num;
+    const size_t MN = static_cast<size_t>(M) * N;
+    for (size_t i = 0; i < MN; ++i) S[i] = cd(echo[2 * i], echo[2 * i + 1]);
+    if (opt->data_type == 1) {
+        swap_halves_rows(S.data(), M, N);
+        fft_rows(S.data(), M, N, +1);
+        for (auto& v : S) v /= static_cast<double>(N);
+        swap_halves_rows(S.data(), M, N);
+    }
+    if (opt->if_hpc && opt->data_type == 2) orc_hpc(reinterpret_cast<double*>(S.data()), velocity, p);
+}
+}  // namespace
+
+int orc_hrrp(const float* echo, const double* velocity, const orc_params* p, const orc_options* opt,
+             double* hrrp) {
+    // getHRRP (common.cu:541-580) on the rows of isar_main.cu:57-67
+    const int M = p->echo_num, N = p->range_num;
+    if (M < 1 || N < 2 || !is_pow2(N) || (opt->data_type != 1 && opt->data_type != 2)) return 1;
+    std::vector<cd> S(static_cast<size_t>(M) * N);
+    front_rows(S, echo, velocity, p, opt);
+    if (opt->data_type == 1) {
+        // IFDS: ifftshift, forward FFT, ifftshift of a copy (common.cu:546-558)
+        swap_halves_rows(S.data(), M, N);
+        fft_rows(S.data(), M, N, -1);
+        swap_halves_rows(S.data(), M, N);
+    } else {
+        // STRETCH: Hamming window over range, forward FFT, ifftshift (common.cu:561-571)
+        const std::vector<double> ham = hamming(N);
+        for (int m = 0; m < M; ++m)
+            for (int n = 0; n < N; ++n) S[static_cast<size_t>(m) * N + n] *= ham[n];
+        fft_rows(S.data(), M, N, -1);
+        swap_halves_rows(S.data(), M, N);
+    }
+    std::memcpy(hrrp, S.data(), S.size() * sizeof(cd));
+    return 0;
+}
+
 int orc_isar_main(const float* echo, const double* velocity, const orc_params* p,
                   const orc_options* opt, float* img, orc_trace* tr) {
     // isar_main.cu:3-227 (isarMainSingle, isar_imaging_export.cpp:208-212)
@@ -664,15 +706,7 @@ int orc_isar_main(const float* echo, const double* velocity, const orc_params* p
     if (M < 2 || M > 1024 || N < W || N < M || W < 2 || !is_pow2(N)) return 1;
     const size_t MN = static_cast<size_t>(M) * N, MW = static_cast<size_t>(M) * W;
     std::vector<cd> S(MN);
-    for (size_t i = 0; i < MN; ++i) S[i] = cd(echo[2 * i], echo[2 * i + 1]);
-    if (opt->data_type == 1) {
-        // :15-27 IFDS pre-stage
-        swap_halves_rows(S.data(), M, N);
-        fft_rows(S.data(), M, N, +1);
-        for (auto& v : S) v /= static_cast<double>(N);
-        swap_halves_rows(S.data(), M, N);
-    }
-    if (opt->if_hpc && opt->data_type == 2) orc_hpc(reinterpret_cast<double*>(S.data()), velocity, p);
+    front_rows(S, echo, velocity, p, opt);
     if (opt->if_hrrp && opt->data_type == 2) {
         // getHRRP, common.cu:563-566: the STRETCH branch windows d_data in place (Q11 kept);
         // the HRRP itself is a side output that the image does not depend on.

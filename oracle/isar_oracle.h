@@ -138,7 +138,7 @@ void orc_mtrc(double* x, const orc_params* p, int data_type);
 
 /* Synthetic-frame-free helpers used by the tests. */
 int orc_next_pow2(int n);                            /* common.cu:521-529 */
-double orc_image_entropy(const float* img, int n);
+double orc_image_entropy(const float* img, int n);   /* -sum p ln p, p = img^2/sum img^2 */
 
 /* IFDS pulse compression of one echo (common.cu:583-604, 635-746): iq = `len` complex samples
  * (interleaved), NFFT = orc_next_pow2(len); the reference chirp of S = (int)(Fs Tp) samples,
@@ -146,7 +146,14 @@ double orc_image_entropy(const float* img, int n);
  * c1 = -2 pi fc v2 / (1 - v2), c2 = pi B / Tp, zero-padded to NFFT; out = IFFT(conj(FFT(ref))
  * FFT(iq)) / NFFT cut to `cut` samples from offset = clamp(i* - cut/2, 0, NFFT - cut), i* the
  * first index of max |Re|+|Im| (cublasIcamax). Returns the offset, or -1 for bad sizes. */
-int orc_pulse_compress(const float* iq, int len, double velocity, const orc_params* p, int cut, double* out);   /* -sum p ln p, p = img^2/sum img^2 */
+int orc_pulse_compress(const float* iq, int len, double velocity, const orc_params* p, int cut, double* out);
+
+/* High-resolution range profiles of one frame (getHRRP, common.cu:541-580, called at
+ * isar_main.cu:57-67): the frame's rows after the IFDS pre-stage or the STRETCH HPC; then for
+ * IFDS ifftshift(FFT(ifftshift(row))), for STRETCH ifftshift(FFT(hamming_N * row)). hrrp: M*N
+ * interleaved complex doubles. Returns 0, or 1 for bad sizes or data type. */
+int orc_hrrp(const float* echo, const double* velocity, const orc_params* p, const orc_options* opt,
+             double* hrrp);
 
 #ifdef __cplusplus
 }

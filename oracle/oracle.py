@@ -266,6 +266,20 @@ def pulse_compress(iq: np.ndarray, velocity: float, p: OrcParams, cut: int = 409
     return out, off
 
 
+def hrrp(echo: np.ndarray, velocity: np.ndarray, p: OrcParams, opt: OrcOptions | None = None) -> np.ndarray:
+    """getHRRP of one frame: complex128 (M, N)."""
+    opt = opt or options()
+    e = np.ascontiguousarray(echo, dtype=np.complex64)
+    v = np.ascontiguousarray(velocity, dtype=np.float64)
+    out = np.zeros((p.echo_num, p.range_num), np.complex128)
+    L = lib()
+    L.orc_hrrp.restype = C.c_int
+    L.orc_hrrp.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(OrcParams), C.POINTER(OrcOptions), C.c_void_p]
+    if L.orc_hrrp(_ptr(e), _ptr(v), C.byref(p), C.byref(opt), _ptr(out)) != 0:
+        raise ValueError("hrrp: unsupported sizes or data type")
+    return out
+
+
 def image_entropy(img: np.ndarray) -> float:
     a = np.ascontiguousarray(img, dtype=np.float32)
     return lib().orc_image_entropy(_ptr(a), a.size)
