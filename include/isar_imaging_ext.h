@@ -135,6 +135,33 @@ ISAR_API int isar_set_hrrp_output(isar_context* ctx, void* d_hrrp);
  * the window's M x N profiles (complex float) to h_hrrp. Synchronous. */
 ISAR_API int isar_session_hrrp(float* h_hrrp);
 
+/* Intermediate arrays of one frame of the last run, as the reference writes them with its
+ * DATA_WRITE_BACK_* compile switches (isar_main.cu:10-12, 51-53, 76-78, 101-103, 152-154,
+ * 178-180, 221-223). The fused kernels never hold most of them whole; they are rebuilt on the
+ * device from what the run kept. The last run must have been a full run, and the caller's echo,
+ * velocity and image buffers must still hold its data. */
+enum isar_dump_stage {
+    ISAR_DUMP_DATAW = 0,   /* dataW.dat: the echo rows, M x N complex                          */
+    ISAR_DUMP_HPC = 1,     /* hpc.dat: after the IFDS pre-stage or the STRETCH HPC, M x N       */
+    ISAR_DUMP_HRRP = 2,    /* hrrp.dat: getHRRP's profiles, M x N complex                       */
+    ISAR_DUMP_RA = 3,      /* ra.dat: aligned and centred range profiles, M x N complex         */
+    ISAR_DUMP_PC = 4,      /* pc.dat: the cut after Doppler tracking and autofocus, M x W       */
+    ISAR_DUMP_MTRC = 5,    /* mtrc.dat: after MTRC (the PC array when MTRC is off), M x W       */
+    ISAR_DUMP_FINAL = 6,   /* final.dat: the image, M x W float                                 */
+    ISAR_DUMP_COUNT = 7
+};
+/* Bytes of one frame's array of `stage`; 0 for a bad stage. */
+ISAR_API size_t isar_stage_bytes(const isar_context* ctx, int stage);
+/* Copies frame `frame`'s array of `stage` to host memory (isar_stage_bytes bytes; complex
+ * arrays as interleaved floats). Synchronous: waits for the device first. */
+ISAR_API int isar_stage_read(isar_context* ctx, int stage, int frame, void* h_out);
+/* Writes the arrays of the stages whose bit is set in `mask` (bit s = stage s) as the reference's
+ * text files <dir>/dataW.dat ... final.dat: one value per line, 5 decimals, complex values as
+ * the real part and then the imaginary part (ioOperation::writeFile, common.cu:1194-1259). */
+ISAR_API int isar_stage_dump(isar_context* ctx, int frame, unsigned mask, const char* dir);
+/* Drop-in session: isar_stage_dump of the last isarMainSingle. */
+ISAR_API int isar_session_dump(unsigned mask, const char* dir);
+
 /* Rows of the range-alignment update handled per workgroup (2, 4, 8 or 16; default 8). The
  * choice changes the summation order of the group energies, not the algorithm. */
 ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows);

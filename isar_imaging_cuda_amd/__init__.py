@@ -48,6 +48,9 @@ class _Trace(C.Structure):
 
 
 # kernel families (enum isar_stage in include/isar_imaging_ext.h)
+# arrays of isar_stage_read (the reference's DATA_WRITE_BACK_* dumps): name -> (id, complex?, columns)
+DUMP_STAGES = {"dataW": 0, "hpc": 1, "hrrp": 2, "ra": 3, "pc": 4, "mtrc": 5, "final": 6}
+
 STAGES = {"ifds_pre": 0, "ra_pair0": 1, "ra_corr": 2, "ra_upd": 3, "center": 4, "cut": 5, "pa_colfft": 6,
           "pa_select": 7, "autofocus": 8, "mtrc_rows": 9, "mtrc_czt": 10, "col_window": 11, "mtrc_image": 12,
           "post_nomtrc": 13, "hrrp": 14}
@@ -229,6 +232,28 @@ class Imager:
         device buffers; asynchronous on `stream`."""
         _check(lib().isar_get_phases(self._h, frames, C.c_void_p(start_ptr or 0), C.c_void_p(final_ptr or 0),
                                      C.c_void_p(stream or 0)), "isar_get_phases")
+
+    def stage(self, name: str, frame: int = 0) -> np.ndarray:
+        """Frame `frame`'s array of a stage of the last full run (DUMP_STAGES): complex64 (M, N)
+        for dataW/hpc/hrrp/ra, complex64 (M, W) for pc/mtrc, float32 (M, W) for final."""
+        L = lib()
+        L.isar_stage_bytes.argtypes = [C.c_void_p, C.c_int]
+        L.isar_stage_bytes.restype = C.c_size_t
+        L.isar_stage_read.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        sid = DUMP_STAGES[name]
+        p = self.params
+        cols = p.range_num if sid <= 3 else p.range_num_cut
+        out = np.zeros((p.echo_num, cols), np.float32 if name == "final" else np.complex64)
+        assert out.nbytes == L.isar_stage_bytes(self._h, sid)
+        _check(L.isar_stage_read(self._h, sid, int(frame), out.ctypes.data_as(C.c_void_p)), "isar_stage_read")
+        return out
+
+    def stage_dump(self, directory: str, frame: int = 0, names=tuple(DUMP_STAGES)) -> None:
+        """The reference's text dumps (dataW.dat ... final.dat) of the named stages."""
+        L = lib()
+        L.isar_stage_dump.argtypes = [C.c_void_p, C.c_int, C.c_uint, C.c_char_p]
+        mask = sum(1 << DUMP_STAGES[n] for n in names)
+        _check(L.isar_stage_dump(self._h, int(frame), mask, str(directory).encode()), "isar_stage_dump")
 
     def set_hrrp_output(self, hrrp_ptr: int | None) -> None:
         """Device buffer (max_frames x M x N complex64) for the range profiles that runs with

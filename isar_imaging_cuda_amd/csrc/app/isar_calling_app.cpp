@@ -3,9 +3,12 @@
 // isarMainSingle } x n -> imagingMemDest. Built against include/isar_imaging_export.h only.
 //
 // usage: isar_calling_app <dir> <window_len> <imaging_stride> <n_images> <out_prefix> [polar] [mtrc] [hpc] [data_type]
-//                         [hrrp]  (defaults: polar_type 1, if_mtrc 1, if_hpc 1, data_type 2, if_hrrp 0)
+//                         [hrrp] [dump_dir]  (defaults: polar_type 1, if_mtrc 1, if_hpc 1, data_type 2,
+//                         if_hrrp 0, no dump)
 // writes <out_prefix>_<i>.bin (raw float32 image, window_len x 512) and <out_prefix>_0.txt; with
-// if_hrrp also <out_prefix>_hrrp_<i>.bin (window_len x range_num complex float range profiles).
+// if_hrrp also <out_prefix>_hrrp_<i>.bin (window_len x range_num complex float range profiles);
+// with dump_dir the first window's intermediate arrays as the reference's text dumps
+// (dataW.dat ... final.dat).
 // n_images = 0 only parses the directory (no GPU call) and prints the decoded headers.
 #include <chrono>
 #include <cstdio>
@@ -18,7 +21,7 @@
 
 int main(int argc, char** argv) {
     if (argc < 6) {
-        std::fprintf(stderr, "usage: %s dir window_len imaging_stride n_images out_prefix [polar] [mtrc] [hpc] [data_type] [hrrp]\n",
+        std::fprintf(stderr, "usage: %s dir window_len imaging_stride n_images out_prefix [polar] [mtrc] [hpc] [data_type] [hrrp] [dump_dir]\n",
                      argv[0]);
         return 2;
     }
@@ -35,6 +38,7 @@ int main(int argc, char** argv) {
     int window_head = 0;
     const int option_alignment = 0, option_phase = 1;
     const bool if_hrrp = argc > 10 ? std::atoi(argv[10]) != 0 : false;
+    const std::string dump_dir = argc > 11 ? argv[11] : "";
 
     vec1D_DBL dataN;
     vec1D_FLT turnAngle;
@@ -82,6 +86,10 @@ int main(int argc, char** argv) {
         std::ofstream out(prefix + "_" + std::to_string(i) + ".bin", std::ios::binary);
         out.write(reinterpret_cast<const char*>(img.data()), static_cast<std::streamsize>(img.size() * sizeof(float)));
         if (i == 0) writeFileFLT(prefix + "_0.txt", img.data(), img.size());
+        if (i == 0 && !dump_dir.empty() && isar_session_dump((1u << ISAR_DUMP_COUNT) - 1, dump_dir.c_str()) != ISAR_OK) {
+            std::fprintf(stderr, "isar_session_dump: %s\n", isar_last_error());
+            return 8;
+        }
         if (if_hrrp) {
             vec1D_FLT hrrp(img.size() / 512 * static_cast<size_t>(data_type == 1 ? 4096 : (frame_len - 256) / 4) * 2);
             if (isar_session_hrrp(hrrp.data()) != ISAR_OK) {

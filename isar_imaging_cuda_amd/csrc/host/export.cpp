@@ -341,6 +341,11 @@ ISAR_API int isar_session_hrrp(float* h_hrrp) {
     return ISAR_OK;
 }
 
+ISAR_API int isar_session_dump(unsigned mask, const char* dir) {
+    if (!g.ctx) return err("isar_session_dump before imagingMemInit"), ISAR_ERR_STATE;
+    return isar_stage_dump(g.ctx, 0, mask, dir);
+}
+
 DLL_EXPORT_API void isarMainSingle(float* h_img, const int& data_type, const int& option_alignment, const int& option_phase,
                                    const bool& if_hrrp, const bool& if_hpc, const bool& if_mtrc) {
     if (!g.ctx || !h_img) { err("isarMainSingle before imagingMemInit"); return; }

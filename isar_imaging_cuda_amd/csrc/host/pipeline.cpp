@@ -298,6 +298,8 @@ int Context::run_frames(int frames, const void* echo, const double* velocity, co
     if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
     last_frames = frames;
     if (mode != ISAR_RUN_BACK) front_frames = frames;
+    last_job = J;
+    last_mode = mode;
     return ISAR_OK;
 }
 
@@ -389,8 +391,6 @@ int Context::trace(int frame, isar_frame_trace* out) {
 // C ABI (isar_imaging_ext.h)
 // ------------------------------------------------------------------------------------------
 using isar::fail;
-
-struct isar_context : isar::Context {};
 
 extern "C" {
 

@@ -53,6 +53,11 @@ struct Context {
     int run_frames(int frames, const void* echo, const double* velocity, const long long* frame_row,
                    const void* phi0, float* img, const long long* frame_img, const isar_options& opt,
                    hipStream_t stream);
+    // the last run, for isar_stage_read: its job (with the caller's buffer pointers) and stage
+    // mode, and scratch for rebuilding intermediate arrays (allocated on the first stage read)
+    Job last_job{};
+    int last_mode = -1;
+    float2* stage_scratch = nullptr;
     float2* hrrp_out = nullptr;           // max_frames x M x N profiles written by if_hrrp runs, or null
     int ra_chunk = kRaChunkRowsDefault;   // tunable: rows per workgroup in the alignment update
     // times every range-alignment chunk size on these frames and keeps the fastest (synchronous)
@@ -67,3 +72,6 @@ size_t workspace_bytes(const isar_radar_params& p, int frames);
 int validate_params(const isar_radar_params& p);
 
 }  // namespace isar
+
+// the C API's opaque context handle
+struct isar_context : isar::Context {};

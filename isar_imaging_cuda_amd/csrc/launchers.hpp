@@ -63,4 +63,11 @@ void launch_image_entropy(int frames, int n, const float* img, double* out, hipS
 // standalone batched FFT used by the tests: n-point transforms of `batch` contiguous rows
 bool debug_fft(int n, int dir, const float2* in, float2* out, int batch, hipStream_t s);
 
+// stage.hip: intermediate arrays of a finished run, rebuilt for inspection
+void launch_stage_hpc(const float2* echo, const double* vel, double hpc_k, int M, int N, float2* out, hipStream_t s);
+void launch_stage_roll(const float2* st, int M, int N, int shift, float2* out, hipStream_t s);
+void launch_stage_rowphase(const float2* x, const float2* phi, int M, int W, float2* out, hipStream_t s);
+void launch_stage_transpose(const float2* in, int R, int C, int sr, int sc, float scale, float2* out, hipStream_t s);
+void launch_stage_mtrc_out(const float2* a, int M, int W, int ifds, float2* out, hipStream_t s);
+
 }  // namespace isar
