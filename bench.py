@@ -110,7 +110,7 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
-def learned_vs_plain(frames: int = 64) -> dict:
+def learned_vs_plain(frames: int = 256) -> dict:
     """Autofocus iterations to reach the plain 120-iteration image entropy (within 1e-5 relative)
     on held-out 256 x 2048 frames (range-aligned, per-pulse phase errors uniform in [-pi, pi],
     15 dB SNR): plain from the Doppler phase versus the shipped warm-start network."""

@@ -174,6 +174,17 @@ __global__ void k_compose_phase(int frames, int M, const float2* __restrict__ ph
     phi_w[i] = a > 0.0f ? cmul(phi_b[i], cscale(q, 1.0f / a)) : phi_b[i];
 }
 
+// Per frame, the starting phase whose image entropy is lower: out = ent_b < ent_a ? phi_b : phi_a
+// (ties keep phi_a).
+__global__ void k_select_phase(int frames, int M, const float2* __restrict__ phi_a, const double* __restrict__ ent_a,
+                               const float2* __restrict__ phi_b, const double* __restrict__ ent_b,
+                               float2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= frames * M) return;
+    const int f = i / M;
+    out[i] = ent_b[f] < ent_a[f] ? phi_b[i] : phi_a[i];
+}
+
 // Image entropy -sum p ln p, p = img^2 / sum img^2, one workgroup per image, FP64 sums.
 __global__ void k_image_entropy(int n, const float* __restrict__ img, double* __restrict__ out) {
     __shared__ double red[256];
@@ -249,6 +260,11 @@ void launch_phase_change(int frames, int M, const float2* a, const float2* b, fl
 }
 void launch_compose_phase(int frames, int M, const float2* phi_b, const float* pred, float2* phi_w, hipStream_t s) {
     ISAR_LAUNCH(k_compose_phase, dim3((frames * M + 255) / 256), dim3(256), 0, s, frames, M, phi_b, pred, phi_w);
+}
+void launch_select_phase(int frames, int M, const float2* phi_a, const double* ent_a, const float2* phi_b,
+                         const double* ent_b, float2* out, hipStream_t s) {
+    ISAR_LAUNCH(k_select_phase, dim3((frames * M + 255) / 256), dim3(256), 0, s, frames, M, phi_a, ent_a, phi_b, ent_b,
+                out);
 }
 void launch_image_entropy(int frames, int n, const float* img, double* out, hipStream_t s) {
     ISAR_LAUNCH(k_image_entropy, dim3(frames), dim3(256), 0, s, n, img, out);

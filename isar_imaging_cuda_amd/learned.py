@@ -69,6 +69,7 @@ def _bind() -> C.CDLL:
     L.isar_phase_change.argtypes = [i, i, vp, vp, vp, i, vp]
     L.isar_compose_phase.argtypes = [i, i, vp, vp, vp, vp]
     L.isar_image_entropy.argtypes = [i, i, vp, vp, vp]
+    L.isar_select_phase.argtypes = [i, i, vp, vp, vp, vp, vp, vp]
     L._learned_bound = True
     return L
 
@@ -147,12 +148,19 @@ def image_entropy(frames: int, n: int, d_img: int, d_out: int, stream: int | Non
     _check(_bind().isar_image_entropy(frames, n, _p(d_img), _p(d_out), _p(stream)), "isar_image_entropy")
 
 
+def select_phase(frames: int, M: int, d_phi_a: int, d_ent_a: int, d_phi_b: int, d_ent_b: int, d_out: int,
+                 stream: int | None = None) -> None:
+    _check(_bind().isar_select_phase(frames, M, _p(d_phi_a), _p(d_ent_a), _p(d_phi_b), _p(d_ent_b), _p(d_out),
+                                     _p(stream)), "isar_select_phase")
+
+
 # ------------------------------------------------------------------------------------------
 # the pipeline with a learned warm start
 # ------------------------------------------------------------------------------------------
 class WarmStartImager:
     """An imaging context plus the network: front (FRONT_ITERS plain iterations) -> network ->
-    back (the iteration from the prediction, then the image)."""
+    judge (keep the prediction only where its image entropy is lower than the plain phase's) ->
+    back (the iteration from the chosen start, then the image)."""
 
     def __init__(self, params: RadarParams, max_frames: int, net_params: np.ndarray | None = None,
                  front_iters: int = FRONT_ITERS):
@@ -173,6 +181,9 @@ class WarmStartImager:
         self.phi_w = torch.zeros_like(self.phis[0])
         self.feat = torch.zeros((max_frames, 2 * self.M * front_iters), dtype=torch.float32, device=dev)
         self.pred = torch.zeros((max_frames, 2 * self.M), dtype=torch.float32, device=dev)
+        self.phi_s = torch.zeros_like(self.phis[0])      # the judged start
+        self.ent_plain = torch.zeros(max_frames, dtype=torch.float64, device=dev)
+        self.ent_warm = torch.zeros(max_frames, dtype=torch.float64, device=dev)
 
     @property
     def phi_last(self):
@@ -199,16 +210,29 @@ class WarmStartImager:
         self.net.forward(K, self.feat.data_ptr(), self.pred.data_ptr(), stream)
         compose_phase(K, self.M, self.phi_last.data_ptr(), self.pred.data_ptr(), self.phi_w.data_ptr(), stream)
 
-    def back(self, K: int, img_ptr: int, iterations: int, stream: int, warm: bool = True) -> None:
-        """Autofocus from phi_w (or, with warm=False, the plain Doppler start) and the image."""
+    def back(self, K: int, img_ptr: int, iterations: int, stream: int, warm: bool = True, phi=None) -> None:
+        """Autofocus from `phi` (default: the judged start phi_s; with warm=False the plain
+        Doppler start) and the image."""
         o = Options(iterations=iterations, stage_mode=2, warm_start=warm)
-        self.imager.run_device(0, 0, img_ptr, K, o, phi0_ptr=self.phi_w.data_ptr() if warm else None,
-                               stream=stream)
+        start = (self.phi_s if phi is None else phi).data_ptr() if warm else None
+        self.imager.run_device(0, 0, img_ptr, K, o, phi0_ptr=start, stream=stream)
+
+    def judge(self, K: int, img_ptr: int, stream: int) -> None:
+        """The plain objective decides: image entropy from phi_F and from phi_w (img_ptr: K x M x W
+        floats of scratch), phi_s = the start with the lower entropy per frame."""
+        n = self.M * self.W
+        self.back(K, img_ptr, 0, stream, phi=self.phi_last)
+        image_entropy(K, n, img_ptr, self.ent_plain.data_ptr(), stream)
+        self.back(K, img_ptr, 0, stream, phi=self.phi_w)
+        image_entropy(K, n, img_ptr, self.ent_warm.data_ptr(), stream)
+        select_phase(K, self.M, self.phi_last.data_ptr(), self.ent_plain.data_ptr(), self.phi_w.data_ptr(),
+                     self.ent_warm.data_ptr(), self.phi_s.data_ptr(), stream)
 
     def run(self, K: int, echo_ptr: int, vel_ptr: int, img_ptr: int, iterations: int, stream: int) -> None:
         """The learned chain: F + `iterations` autofocus iterations in total."""
         self.front(K, echo_ptr, vel_ptr, stream)
         self.predict(K, stream)
+        self.judge(K, img_ptr, stream)
         self.back(K, img_ptr, iterations, stream)
 
 
@@ -304,12 +328,14 @@ def iterations_to_reach(ent: np.ndarray, target: np.ndarray) -> np.ndarray:
 def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int = 4, max_iter: int = 120,
              log=print, front_iters: int = FRONT_ITERS) -> dict:
     """Iterations until the image entropy is within 1e-5 relative of the plain 120-iteration
-    result, plain (from phi0) versus learned (FRONT_ITERS iterations, the network, then the
-    iteration from the prediction), on the frames of `seeds`. A learned frame that never reaches the plain
-    entropy within max_iter iterations falls back to the plain result and is charged both."""
+    result, plain (from phi0) versus learned (FRONT_ITERS iterations, the network, the entropy
+    judgement, then the iteration from the judged start), on the frames of `seeds`. The learned
+    chain shares the plain front iterations: a frame the plain iteration settles within them counts
+    the same for both. A learned frame that never reaches the plain entropy within max_iter
+    iterations falls back to the plain result and is charged both."""
     import torch
     dev = torch.device("cuda:0")
-    plain_it, learned_it, accepted, e_plain_all, e_learned_all = [], [], [], [], []
+    plain_it, learned_it, accepted, e_plain_all, e_learned_all, kept = [], [], [], [], [], []
     ws = None
     for c0 in range(0, len(seeds), chunk):
         part = seeds[c0:c0 + chunk]
@@ -324,6 +350,8 @@ def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int
         s = torch.cuda.current_stream(dev).cuda_stream
         ws.front(K, e.data_ptr(), v.data_ptr(), s)
         ws.predict(K, s)
+        ws.judge(K, img.data_ptr(), s)
+        kept.append((ws.ent_warm[:K] < ws.ent_plain[:K]).cpu().numpy())
         curve_p = np.zeros((K, max_iter + 1))
         curve_l = np.zeros((K, max_iter + 1))
         for k in range(max_iter + 1):
@@ -339,7 +367,7 @@ def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int
         il = iterations_to_reach(curve_l, target)
         ok = il <= max_iter
         plain_it.append(ip)
-        learned_it.append(np.where(ok, ws.F + il, ws.F + max_iter + ip))
+        learned_it.append(np.where(ip <= ws.F, ip, np.where(ok, ws.F + il, ws.F + max_iter + ip)))
         accepted.append(ok)
         e_plain_all.append(e_plain)
         e_learned_all.append(np.where(ok, curve_l[np.arange(K), np.minimum(il, max_iter)], e_plain))
@@ -351,6 +379,7 @@ def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int
             "learned_mean_iterations": float(il.mean()),
             "iteration_ratio": float(ip.mean() / max(il.mean(), 1e-9)),
             "accepted_fraction": float(acc.mean()),
+            "prediction_kept_fraction": float(np.concatenate(kept).mean()),
             "plain_median": float(np.median(ip)), "learned_median": float(np.median(il)),
             "entropy_rel_diff_max": float(np.max((np.concatenate(e_learned_all) - np.concatenate(e_plain_all))
                                                  / np.concatenate(e_plain_all)))}

@@ -156,6 +156,17 @@ def test_phase_helpers_match_numpy():
     q = pred.view(np.complex64)
     ref = np.where(np.abs(q) > 0, b * q / np.where(np.abs(q) > 0, np.abs(q), 1), b)
     assert np.abs(w.cpu().numpy().view(np.complex64) - ref).max() < 2e-6
+    # the entropy judgement: frame k takes b only where its entropy is strictly lower
+    ea = np.array([1.0, 2.0, 3.0, 4.0, 5.0])
+    eb = np.array([0.5, 2.0, 3.5, 3.0, 5.0 - 1e-12])
+    dea = torch.from_numpy(ea).to(dev)
+    deb = torch.from_numpy(eb).to(dev)
+    sel = torch.zeros((K, 2 * M), dtype=torch.float32, device=dev)
+    L.select_phase(K, M, da.data_ptr(), dea.data_ptr(), db.data_ptr(), deb.data_ptr(), sel.data_ptr(), s)
+    torch.cuda.synchronize()
+    got = sel.cpu().numpy().view(np.complex64)
+    for k, take_b in enumerate([True, False, False, True, True]):
+        assert np.array_equal(got[k], b[k] if take_b else a[k]), f"frame {k}"
 
 
 @pytest.mark.gpu
