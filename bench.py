@@ -160,6 +160,93 @@ def subapertures(count: int = 64, replays: int = 10) -> dict:
             "ra_chunk_rows": {int(m): int(c) for m, c in zip(plan.group_pulses, chosen)}}
 
 
+def large_batch(frames: int = 8, distinct: int = 2, steps: int = 3) -> dict:
+    """8 frames of 1024 x 16384 complex64 (128 MiB each; 1000 scatterers, 7 km/s, 100 m/s^2,
+    0.08 rad/s) through the full plain pipeline as one device batch; `distinct` synthetic frames
+    repeated to fill the batch."""
+    import torch
+
+    import isar_imaging_cuda_amd as isar
+    from isar_imaging_cuda_amd import synth
+
+    cfgs = [synth.preset(5, seed=0x5EED + k) for k in range(distinct)]
+    data = [synth.frame(c) for c in cfgs]
+    h = synth.radar_header(cfgs[0])
+    M5, N5 = data[0][0].shape
+    dev = torch.device("cuda:0")
+    echo = torch.from_numpy(np.stack([data[k % distinct][0] for k in range(frames)]).view(np.float32)).to(dev)
+    vel = torch.from_numpy(np.stack([data[k % distinct][1] for k in range(frames)])).to(dev)
+    img = torch.empty((frames, M5, W), dtype=torch.float32, device=dev)
+    im = isar.Imager(isar.RadarParams(M5, N5, h["band_width"], h["fc"], h["Tp"], h["Fs"]), max_frames=frames)
+    s = torch.cuda.current_stream(dev)
+    run = lambda: im.run_device(echo.data_ptr(), vel.data_ptr(), img.data_ptr(), frames, isar.Options(),
+                                stream=s.cuda_stream)
+    run()
+    torch.cuda.synchronize()
+    U = float(np.mean([im.trace(k).unit2_num for k in range(distinct)]))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        run()
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    fps = frames / (ms / 1e3)
+    fb = algorithmic_bytes_per_frame(M5, N5, W, U)
+    im.close()
+    del echo, vel, img
+    torch.cuda.empty_cache()
+    return {"workload": f"{frames} frames of {M5} x {N5} complex64 ({distinct} distinct), full plain pipeline",
+            "ms_per_batch": round(ms, 3), "frames_per_s": round(fps, 1), "mean_selected_bins": U,
+            "algorithmic_bytes_per_frame": fb, "achieved_GBs": round(fb * fps / 1e9, 1),
+            "frac": round(fb * fps / 1e9 / HBM_PEAK_GBS, 4)}
+
+
+def single_frame(params, echo, vel, stream, runs: int = 20) -> dict:
+    """Latency of one frame alone (a batch of one, device-resident), the drop-in's unit of work."""
+    import torch
+
+    import isar_imaging_cuda_amd as isar
+    im = isar.Imager(params, max_frames=1)
+    img1 = torch.empty((1, params.echo_num, W), dtype=torch.float32, device=echo.device)
+    run = lambda: im.run_device(echo.data_ptr(), vel.data_ptr(), img1.data_ptr(), 1, isar.Options(),
+                                stream=stream.cuda_stream)
+    for _ in range(3):
+        run()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(runs):
+        run()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    im.close()
+    return {"ms_per_frame": round(e0.elapsed_time(e1) / runs, 4), "runs": runs}
+
+
+def host_transfer(echo, img, ms_per_step: float, stream) -> dict:
+    """The drop-in boundary hands host buffers over: H2D of one step's echo batch and D2H of its
+    images through pinned host memory, timed apart from the device-resident figure."""
+    import torch
+    h_echo = torch.empty(echo.shape, dtype=echo.dtype, pin_memory=True)
+    h_img = torch.empty(img.shape, dtype=img.dtype, pin_memory=True)
+    h_echo.copy_(echo)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        ev[0].record(stream)
+        echo.copy_(h_echo, non_blocking=True)         # the same data back into the batch buffer
+        ev[1].record(stream)
+        h_img.copy_(img, non_blocking=True)
+        ev[2].record(stream)
+    torch.cuda.synchronize()
+    h2d, d2h = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    K = echo.shape[0]
+    return {"h2d_echo_ms": round(h2d, 3), "h2d_GBs": round(echo.numel() * 4 / (h2d / 1e3) / 1e9, 1),
+            "d2h_image_ms": round(d2h, 3), "d2h_GBs": round(img.numel() * 4 / (d2h / 1e3) / 1e9, 1),
+            "pcie_inclusive_frames_per_s": round(K / ((h2d + ms_per_step + d2h) / 1e3), 1)}
+
+
 def load_traffic(kernel: str, frames: int) -> dict | None:
     """HBM traffic per launch of `kernel` at this workload from a committed PMC summary
     (profiles/traffic_<kernel>_256x4096.json, written from two rocprofv3 --pmc runs and scaled by
@@ -189,6 +276,9 @@ def main() -> int:
                          "the size the committed traffic measurement used)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain iteration count")
     ap.add_argument("--no-subapertures", action="store_true", help="skip the sub-aperture plan measurement")
+    ap.add_argument("--no-large", action="store_true", help="skip the 8 x 1024 x 16384 batch")
+    ap.add_argument("--large-only", action="store_true", help="only the 8 x 1024 x 16384 batch (for profiling)")
+    ap.add_argument("--no-transfer", action="store_true", help="skip the host-transfer (PCIe) timing")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -198,6 +288,9 @@ def main() -> int:
         return 2
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
+    if args.large_only:
+        print(json.dumps({"large_batch": large_batch()}))
+        return 0
 
     import torch
 
@@ -296,9 +389,14 @@ def main() -> int:
                               "achieved_GBs": round(frame_bytes * fps / 1e9, 2),
                               "frac": round(frame_bytes * fps / 1e9 / HBM_PEAK_GBS, 5)},
     }
+    if not args.no_transfer:
+        out["host_transfer"] = host_transfer(echo, img, ms_per_step, stream)
+    out["single_frame"] = single_frame(params, echo, vel, stream)
     im.close()
     del echo, vel, img
     torch.cuda.empty_cache()
+    if not args.no_large:
+        out["large_batch"] = large_batch()
     if not args.no_learned:
         out["learned_vs_plain"] = learned_vs_plain()
     if not args.no_subapertures:

@@ -162,6 +162,14 @@ ISAR_API int isar_stage_dump(isar_context* ctx, int frame, unsigned mask, const 
 /* Drop-in session: isar_stage_dump of the last isarMainSingle. */
 ISAR_API int isar_session_dump(unsigned mask, const char* dir);
 
+/* Form of the fast-entropy autofocus. FUSED runs each frame's 120 iterations in one workgroup,
+ * which fills the GPU when a batch has hundreds of frames; SPLIT spreads each frame's selected
+ * range bins over several workgroups with two launches per iteration, for small batches and
+ * single frames. AUTO (default) splits batches of at most 32 frames. Both forms give the same
+ * phases bit for bit. */
+enum isar_autofocus_mode { ISAR_AF_AUTO = 0, ISAR_AF_FUSED = 1, ISAR_AF_SPLIT = 2 };
+ISAR_API int isar_set_autofocus_mode(isar_context* ctx, int mode);
+
 /* Rows of the range-alignment update handled per workgroup (2, 4, 8 or 16; default 8). The
  * choice changes the summation order of the group energies, not the algorithm. */
 ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows);

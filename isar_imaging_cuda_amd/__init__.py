@@ -48,6 +48,9 @@ class _Trace(C.Structure):
 
 
 # kernel families (enum isar_stage in include/isar_imaging_ext.h)
+# isar_autofocus_mode
+AF_AUTO, AF_FUSED, AF_SPLIT = 0, 1, 2
+
 # arrays of isar_stage_read (the reference's DATA_WRITE_BACK_* dumps): name -> (id, complex?, columns)
 DUMP_STAGES = {"dataW": 0, "hpc": 1, "hrrp": 2, "ra": 3, "pc": 4, "mtrc": 5, "final": 6}
 
@@ -254,6 +257,13 @@ class Imager:
         L.isar_stage_dump.argtypes = [C.c_void_p, C.c_int, C.c_uint, C.c_char_p]
         mask = sum(1 << DUMP_STAGES[n] for n in names)
         _check(L.isar_stage_dump(self._h, int(frame), mask, str(directory).encode()), "isar_stage_dump")
+
+    def set_autofocus_mode(self, mode: int) -> None:
+        """AF_AUTO (split batches of at most 32 frames), AF_FUSED (one workgroup per frame) or
+        AF_SPLIT (several workgroups per frame); the phases are the same bit for bit."""
+        L = lib()
+        L.isar_set_autofocus_mode.argtypes = [C.c_void_p, C.c_int]
+        _check(L.isar_set_autofocus_mode(self._h, int(mode)), "isar_set_autofocus_mode")
 
     def set_hrrp_output(self, hrrp_ptr: int | None) -> None:
         """Device buffer (max_frames x M x N complex64) for the range profiles that runs with

@@ -1,7 +1,7 @@
 // emu_driver.cpp — runs the kernel sources on the CPU (hip_emu.hpp) through the same C ABI as
 // the device library. Test infrastructure: built with AddressSanitizer by the CPU test suite.
 //
-// usage: isar_emu_driver <input.bin> <output.bin>
+// usage: isar_emu_driver <input.bin> <output.bin> [autofocus_mode]
 // input : int32 M, N, W, K; int64 band_width, fc; double Tp; int64 Fs; isar_options;
 //         echo K*M*N complex float; velocity K*M double; [phi0 K*M complex float if warm_start]
 // output: img K*M*W float; then per frame: int32[10] scalars, float[2], lag[M-1], imax[M-1],
@@ -13,8 +13,8 @@
 #include "isar_imaging_ext.h"
 
 int main(int argc, char** argv) {
-    if (argc != 3) {
-        std::fprintf(stderr, "usage: %s input.bin output.bin\n", argv[0]);
+    if (argc != 3 && argc != 4) {
+        std::fprintf(stderr, "usage: %s input.bin output.bin [autofocus_mode]\n", argv[0]);
         return 2;
     }
     FILE* in = std::fopen(argv[1], "rb");
@@ -38,6 +38,10 @@ int main(int argc, char** argv) {
     isar_context* ctx = nullptr;
     if (isar_create(&p, K, &ctx) != 0) {
         std::fprintf(stderr, "create: %s\n", isar_last_error());
+        return 6;
+    }
+    if (argc == 4 && isar_set_autofocus_mode(ctx, std::atoi(argv[3])) != 0) {
+        std::fprintf(stderr, "autofocus mode: %s\n", isar_last_error());
         return 6;
     }
     std::vector<float> img(static_cast<size_t>(K) * M * W);

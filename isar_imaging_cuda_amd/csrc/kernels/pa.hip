@@ -170,6 +170,11 @@ struct AfCfg {
     static constexpr int CF = NT / TPF;                       // columns per chunk
     static constexpr int RPT = (M + NT - 1) / NT;             // rows per thread when M > NT
     static constexpr int Q = M <= NT ? (NT / M >= 16 ? 16 : NT / M >= 8 ? 8 : NT / M >= 4 ? 4 : NT / M >= 2 ? 2 : 1) : 1;
+    // Summation order of the per-pulse sums: the selected bins in sub-chunks of SUB columns, each
+    // summed column by column, the sub-chunk sums added in order. The one-workgroup form and the
+    // split form (k_af_cols / k_af_phase) both follow it.
+    static constexpr int SUB = CF / Q;
+    static_assert(CF % Q == 0, "sub-chunks must tile a chunk");
 };
 
 template <int M>
@@ -247,13 +252,16 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
             __syncthreads();
             const int ncol = min(CF, U - c0);
             if constexpr (Q > 1) {
-                // Q lanes per row, fixed summation order
+                // Q lanes per row: lane q sums sub-chunk q column by column, then every lane adds
+                // the Q sub-chunk sums in order
                 const int m = tid / Q, q = tid % Q;
                 float2 t = make_float2(0.0f, 0.0f);
-                if (m < M)
-                    for (int k = q; k < ncol; k += Q) t = cadd(t, bufs[k * lds_padded_size(M) + m]);
-                t = group_sum2<Q>(t);
-                if (m < M) acc[0] = cadd(acc[0], t);
+                if (m < M) {
+                    const int k1 = min((q + 1) * A::SUB, ncol);
+                    for (int k = q * A::SUB; k < k1; ++k) t = cadd(t, bufs[k * lds_padded_size(M) + m]);
+                }
+#pragma unroll
+                for (int j = 0; j < Q; ++j) acc[0] = cadd(acc[0], make_float2(__shfl(t.x, j, Q), __shfl(t.y, j, Q)));
             } else {
 #pragma unroll
                 for (int i = 0; i < RPT; ++i) {
@@ -288,8 +296,75 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     for (int m = tid; m < M; m += NT) J.phi[static_cast<size_t>(f) * M + m] = phi[m];
 }
 
+// The same fixed-point update with a frame spread over several workgroups, for batches too small
+// to fill the GPU with one workgroup per frame. k_autofocus with no iterations gathers the
+// selected bins and forms the starting phase; then each iteration is k_af_cols, where workgroup
+// (j, f) transforms sub-chunk j of frame f's selected bins and writes its per-pulse sums of
+// conj(S) IFFT(G |G|) / M, and k_af_phase, which adds the sub-chunk sums in order and normalises.
+// The transforms and the summation order are those of k_autofocus, so the phases agree bit for bit.
+template <int M>
+__global__ __launch_bounds__(AfCfg<M>::SUB * AfCfg<M>::TPF) void k_af_cols(Job J) {
+    using A = AfCfg<M>;
+    using C = ColCfg<M>;
+    constexpr int E = A::E, TPF = A::TPF, SUB = A::SUB, NTS = SUB * TPF;
+    HIP_DYNAMIC_SHARED(float2, bufs);   // SUB * lds_padded_size(M)
+    const int j = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const int U = J.dec[f].unit2_num;
+    const int c0 = j * SUB;
+    if (c0 >= U) return;   // the whole workgroup leaves before any barrier
+    const int W = J.W;
+    const int g = tid / TPF, lane = tid % TPF;
+    const int u = c0 + g;
+    const bool valid = u < U;
+    const float2* col = J.su + static_cast<size_t>(f) * W * M + static_cast<size_t>(valid ? u : 0) * M;
+    const float2* phi = J.phi + static_cast<size_t>(f) * M;
+    float2* buf = bufs + g * lds_padded_size(M);
+    const float invM = 1.0f / static_cast<float>(M);
+    const FftTwiddles<M, E> tw = fft_twiddles<M, E>(lane);
+    float2 s[E], x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) s[e] = valid ? col[C::in(lane, e)] : make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = cmul(phi[C::in(lane, e)], s[e]);
+    fft<M, E, -1>(x, buf, lane, tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabsf_(x[e]));
+    relayout_out_to_in<M, E>(x, buf, lane);
+    fft<M, E, +1>(x, buf, lane, tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int pos = C::out(lane, e);
+        const float2 sv = C::same_layout ? s[e] : (valid ? col[pos] : make_float2(0.0f, 0.0f));
+        buf[pos] = cscale(cconjmul(sv, x[e]), invM);
+    }
+    __syncthreads();
+    const int ncol = min(SUB, U - c0);
+    float2* P = J.af_part + (static_cast<size_t>(f) * (W / SUB) + j) * M;
+    for (int m = tid; m < M; m += NTS) {
+        float2 t = make_float2(0.0f, 0.0f);
+        for (int k = 0; k < ncol; ++k) t = cadd(t, bufs[k * lds_padded_size(M) + m]);
+        P[m] = t;
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_af_phase(Job J) {
+    using A = AfCfg<M>;
+    const int f = blockIdx.y, m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= M) return;
+    const int U = J.dec[f].unit2_num;
+    const int nsub = (U + A::SUB - 1) / A::SUB;
+    const float2* P = J.af_part + static_cast<size_t>(f) * (J.W / A::SUB) * M + m;
+    float2 acc = make_float2(0.0f, 0.0f);
+    for (int j = 0; j < nsub; ++j) acc = cadd(acc, P[static_cast<size_t>(j) * M]);
+    const float a = cabsf_(acc);
+    J.phi[static_cast<size_t>(f) * M + m] = a > 0.0f ? cscale(acc, 1.0f / a) : make_float2(0.0f, 0.0f);
+}
+
 // ------------------------------------------------------------------------------------------
 namespace {
+template <int M>
+size_t af_cols_lds() { return static_cast<size_t>(AfCfg<M>::SUB) * lds_padded_size(M) * sizeof(float2); }
 template <int M>
 size_t colfft_lds() {
     using C = ColCfg<M>;
@@ -332,7 +407,8 @@ hipError_t configure_pa_kernels() {
     };
 #define ISAR_SET_PA(MM)                                                          \
     set(reinterpret_cast<const void*>(&k_pa_colfft<MM>), colfft_lds<MM>());      \
-    set(reinterpret_cast<const void*>(&k_autofocus<MM>), autofocus_lds<MM>());
+    set(reinterpret_cast<const void*>(&k_autofocus<MM>), autofocus_lds<MM>());   \
+    set(reinterpret_cast<const void*>(&k_af_cols<MM>), af_cols_lds<MM>());
     ISAR_SET_PA(64) ISAR_SET_PA(128) ISAR_SET_PA(192) ISAR_SET_PA(256) ISAR_SET_PA(384) ISAR_SET_PA(512)
     ISAR_SET_PA(1024)
 #undef ISAR_SET_PA
@@ -349,6 +425,18 @@ void launch_pa_select(const Job& J, hipStream_t s) {
 }
 void launch_autofocus(const Job& J, hipStream_t s) {
     ISAR_PULSE_SWITCH(J.M, ISAR_LAUNCH(k_autofocus<MM>, dim3(J.K), dim3(AfCfg<MM>::NT), autofocus_lds<MM>(), s, J));
+}
+void launch_autofocus_split(const Job& J, hipStream_t s) {
+    Job J0 = J;
+    J0.iterations = 0;
+    ISAR_PULSE_SWITCH(J.M, {
+        ISAR_LAUNCH(k_autofocus<MM>, dim3(J.K), dim3(AfCfg<MM>::NT), autofocus_lds<MM>(), s, J0);
+        constexpr int SUB = AfCfg<MM>::SUB;
+        for (int it = 0; it < J.iterations; ++it) {
+            ISAR_LAUNCH(k_af_cols<MM>, dim3(J.W / SUB, J.K), dim3(SUB * AfCfg<MM>::TPF), af_cols_lds<MM>(), s, J);
+            ISAR_LAUNCH(k_af_phase<MM>, dim3((MM + 255) / 256, J.K), dim3(256), 0, s, J);
+        }
+    });
 }
 
 }  // namespace isar

@@ -17,6 +17,7 @@ constexpr int kRangeCut = 512;        // RANGE_NUM_CUT (common.cuh:71)
 constexpr int kRangeNumIfdsPc = 4096; // RANGE_NUM_IFDS_PC (common.cuh:72)
 constexpr int kRaChunkRowsDefault = 8;   // rows per workgroup in the alignment update kernel
 constexpr int kCutBandRows = 16;      // rows per workgroup when cutting the range window
+constexpr int kAfSplitFrames = 32;    // batches of at most this many frames split the autofocus
 
 // Per-frame scalar decisions written by the device (one record per frame).
 struct FrameDecisions {
@@ -75,6 +76,7 @@ struct Job {
     float2* stm;                  // K x M x W (MTRC range-frequency rows)
     float2* czt;                  // K x M x W (chirp-z output)
     float2* colsum;               // K x W
+    float2* af_part;              // K x (W/8) x M: per-pulse sums per sub-chunk of selected bins (split autofocus)
     const float2* czt_table;      // W x F (FFT of the Bluestein chirp per range bin)
     const float2* czt_pre;        // W x M: exp(j pi a_n m (M - m) / M), the input chirp
     const float2* czt_post;       // W x M: exp(-j pi a_n k^2 / M) / F, the output chirp

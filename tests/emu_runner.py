@@ -52,8 +52,9 @@ class EmuFrame:
 
 
 def run(echo: np.ndarray, velocity: np.ndarray, radar: dict, opts: dict | None = None, phi0=None,
-        timeout: float = 900.0) -> list[EmuFrame]:
-    """echo: complex64 (K, M, N); velocity: (K, M). radar: band_width, fc, Tp, Fs."""
+        timeout: float = 900.0, autofocus_mode: int | None = None) -> list[EmuFrame]:
+    """echo: complex64 (K, M, N); velocity: (K, M). radar: band_width, fc, Tp, Fs.
+    autofocus_mode: isar_autofocus_mode (0 auto, 1 one workgroup per frame, 2 split)."""
     if not DRIVER.exists():
         raise RuntimeError(f"{DRIVER} missing: build with isar_imaging_cuda_amd/build.py --emu")
     opts = opts or default_options()
@@ -75,7 +76,8 @@ def run(echo: np.ndarray, velocity: np.ndarray, radar: dict, opts: dict | None =
             f.write(vel.tobytes())
             if opts.get("warm_start"):
                 f.write(np.ascontiguousarray(phi0, dtype=np.complex64).tobytes())
-        r = subprocess.run([str(DRIVER), str(fin), str(fout)], capture_output=True, text=True, timeout=timeout)
+        cmd = [str(DRIVER), str(fin), str(fout)] + ([] if autofocus_mode is None else [str(autofocus_mode)])
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError(f"emu driver failed rc={r.returncode}\n{r.stdout}\n{r.stderr[-4000:]}")
         raw = fout.read_bytes()

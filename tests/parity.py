@@ -1,6 +1,6 @@
 """Comparison of a pipeline run (device library or CPU emulation) against the FP64 oracle.
 
-Bars (north star): integer decisions identical (near-ties logged, not failed), fractional
+Bars: integer decisions identical (near-ties logged, not failed), fractional
 range-alignment lags within 1e-3 cell, image relative L2 <= 1e-4, image entropy within 1e-5
 relative.
 """

@@ -90,6 +90,31 @@ def test_pipeline_variants(name, dev, orc):
     assert rep.ok(), str(rep)
 
 
+@pytest.mark.parametrize("M,N", [(64, 1024), (128, 1024), (192, 1024), (256, 2048), (384, 1024), (512, 1024),
+                                 (1024, 2048)])
+def test_autofocus_forms_agree(M, N):
+    """One workgroup per frame and the split form (several workgroups per frame, two launches per
+    iteration) give the same phases and images bit for bit."""
+    cfgs = [synth.preset(1, echo_num=M, range_num=N, seed=300 + k) for k in range(2)]
+    frames = [synth.frame(c) for c in cfgs]
+    radar = synth.radar_header(cfgs[0])
+    echo = np.stack([f[0] for f in frames])
+    vel = np.stack([f[1] for f in frames])
+    print(f"[case] autofocus forms M={M} N={N}", flush=True)
+    params = isar.RadarParams(M, N, radar["band_width"], radar["fc"], radar["Tp"], radar["Fs"])
+    out = {}
+    for mode in (isar.AF_FUSED, isar.AF_SPLIT):
+        with isar.Imager(params, max_frames=2) as im:
+            im.set_autofocus_mode(mode)
+            img = im.run_host(echo, vel, isar.Options())
+            out[mode] = (img, [im.trace(k) for k in range(2)])
+    (img_f, tf), (img_s, ts) = out[isar.AF_FUSED], out[isar.AF_SPLIT]
+    for k in range(2):
+        assert tf[k].unit2_num == ts[k].unit2_num
+        assert np.array_equal(tf[k].phi, ts[k].phi), f"frame {k}: phases differ"
+    assert np.array_equal(img_f, img_s)
+
+
 def test_batch_matches_single_frames():
     cfgs = [synth.preset(1, seed=100 + k) for k in range(3)]
     frames = [synth.frame(c) for c in cfgs]
