@@ -29,7 +29,44 @@ __device__ void bitonic_sort_f(float* a, int n) {
 }
 }  // namespace
 
-// One workgroup per frame. LDS: arp[N], sorted[N], reduction scratch.
+// Rows of arp_part / max_part holding partials, in summation order: row 0 (rows 0-1) and, at
+// every merge stride s >= 2, the chunks of min(s, ra_chunk) rows starting at s (the second group
+// of pair 0), stored at head row / 2. At most M/2 rows.
+__device__ int arp_partial_rows(const Job& J, int* rows) {
+    int c = 0;
+    rows[c++] = 0;
+    for (int s = 2; s < J.M; s *= 2)
+        for (int h = s; h < min(2 * s, J.M); h += min(s, J.ra_chunk)) rows[c++] = h / 2;
+    return c;
+}
+
+// Column sums of |S| over all M rows (the ARP numerator, range_alignment.cu:381-399) from the
+// partials range alignment wrote as rows finished moving, added in the order of their rows; one
+// thread per range cell, eight independent loads in flight. The sums go to the frame's first egrp
+// row, free once range alignment has finished.
+__global__ __launch_bounds__(256) void k_arp(Job J) {
+    HIP_DYNAMIC_SHARED(int, rows);   // M/2 + 1
+    const int f = blockIdx.y, n = blockIdx.x * 256 + threadIdx.x;
+    const int M = J.M, N = J.N;
+    if (threadIdx.x == 0) rows[M / 2] = arp_partial_rows(J, rows);
+    __syncthreads();
+    if (n >= N) return;
+    const float* ap = J.arp_part + static_cast<size_t>(f) * (M / 2) * N + n;
+    const int P = rows[M / 2];
+    float acc = 0.0f;
+    int i = 0;
+    for (; i + 8 <= P; i += 8) {
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = ap[static_cast<size_t>(rows[i + q]) * N];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc += v[q];
+    }
+    for (; i < P; ++i) acc += ap[static_cast<size_t>(rows[i]) * N];
+    J.egrp[static_cast<size_t>(f) * M * N + n] = acc;
+}
+
+// One workgroup per frame. LDS: arp[N], sorted[N], reduction scratch, partial-row list.
 __global__ __launch_bounds__(kCenterThreads) void k_center(Job J) {
     HIP_DYNAMIC_SHARED(float, sm);
     constexpr int NT = kCenterThreads;
@@ -39,27 +76,22 @@ __global__ __launch_bounds__(kCenterThreads) void k_center(Job J) {
     float* srt = sm + N;
     float* redv = srt + N;
     int* redi = reinterpret_cast<int*>(redv + 32);
-    // column sums of |S| and max |S| arrive as partials per chunk of rows that finished moving:
-    // rows 0-1 (head 0) and, at every merge stride s >= 2, chunks of min(s, ra_chunk) rows
-    // starting at s (the second group of pair 0). Partials are stored at head row / 2.
-    const float* ap = J.arp_part + static_cast<size_t>(f) * (M / 2) * N;
+    int* rows = redi + 32;   // M/2 + 1
+    // max |S| arrives as partials per chunk of rows that finished moving (arp_partial_rows), the
+    // column sums of |S| from k_arp
     const float* mp = J.max_part + static_cast<size_t>(f) * (M / 2);
+    const float* colsum = J.egrp + static_cast<size_t>(f) * M * N;
+    if (tid == 0) rows[M / 2] = arp_partial_rows(J, rows);
+    __syncthreads();
 
     // H = |S| / max|S| ; ARP = column sums / M  (range_alignment.cu:381-399, all M rows)
     float gm = 0.0f;
-    if (tid == 0) {
-        gm = mp[0];
-        for (int s = 2; s < M; s *= 2)
-            for (int h = s; h < min(2 * s, M); h += min(s, J.ra_chunk)) gm = fmaxf(gm, mp[h / 2]);
-    }
+    for (int i = tid; i < rows[M / 2]; i += NT) gm = fmaxf(gm, mp[rows[i]]);
     gm = block_max<NT>(gm, redv);
     const float inv = gm > 0.0f ? 1.0f / gm : 0.0f;
     const float invM = 1.0f / static_cast<float>(M);
     for (int n = tid; n < N; n += NT) {
-        float acc = ap[n];
-        for (int s = 2; s < M; s *= 2)
-            for (int h = s; h < min(2 * s, M); h += min(s, J.ra_chunk)) acc += ap[static_cast<size_t>(h / 2) * N + n];
-        const float v = acc * inv * invM;
+        const float v = colsum[n] * inv * invM;
         arp[n] = v;
         srt[n] = v;
     }
@@ -189,7 +221,7 @@ __global__ __launch_bounds__(W) void k_cut(Job J) {
 
 hipError_t configure_center_kernels() {
 #ifndef ISAR_EMU
-    const size_t bytes = (2 * 16384 + 64) * sizeof(float);
+    const size_t bytes = (2 * 16384 + 64) * sizeof(float) + (1024 / 2 + 1) * sizeof(int);
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_center), hipFuncAttributeMaxDynamicSharedMemorySize,
                                        static_cast<int>(bytes));
     if (e != hipSuccess) report_config_error(__FILE__, bytes);
@@ -200,7 +232,9 @@ hipError_t configure_center_kernels() {
 }
 
 void launch_center(const Job& J, hipStream_t s) {
-    const size_t lds = (2 * static_cast<size_t>(J.N) + 64) * sizeof(float);
+    const size_t rows = (static_cast<size_t>(J.M) / 2 + 1) * sizeof(int);
+    ISAR_LAUNCH(k_arp, dim3((J.N + 255) / 256, J.K), dim3(256), rows, s, J);
+    const size_t lds = (2 * static_cast<size_t>(J.N) + 64) * sizeof(float) + rows;
     ISAR_LAUNCH(k_center, dim3(J.K), dim3(kCenterThreads), lds, s, J);
 }
 
