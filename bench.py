@@ -279,6 +279,7 @@ def main() -> int:
     ap.add_argument("--no-large", action="store_true", help="skip the 8 x 1024 x 16384 batch")
     ap.add_argument("--large-only", action="store_true", help="only the 8 x 1024 x 16384 batch (for profiling)")
     ap.add_argument("--no-transfer", action="store_true", help="skip the host-transfer (PCIe) timing")
+    ap.add_argument("--no-single", action="store_true", help="skip the single-frame latency")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -391,7 +392,8 @@ def main() -> int:
     }
     if not args.no_transfer:
         out["host_transfer"] = host_transfer(echo, img, ms_per_step, stream)
-    out["single_frame"] = single_frame(params, echo, vel, stream)
+    if not args.no_single:
+        out["single_frame"] = single_frame(params, echo, vel, stream)
     im.close()
     del echo, vel, img
     torch.cuda.empty_cache()
