@@ -278,6 +278,7 @@ def main() -> int:
     ap.add_argument("--no-subapertures", action="store_true", help="skip the sub-aperture plan measurement")
     ap.add_argument("--no-large", action="store_true", help="skip the 8 x 1024 x 16384 batch")
     ap.add_argument("--large-only", action="store_true", help="only the 8 x 1024 x 16384 batch (for profiling)")
+    ap.add_argument("--subapertures-only", action="store_true", help="only the sub-aperture plan (for profiling)")
     ap.add_argument("--no-transfer", action="store_true", help="skip the host-transfer (PCIe) timing")
     ap.add_argument("--no-single", action="store_true", help="skip the single-frame latency")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -291,6 +292,9 @@ def main() -> int:
         return copy_calibration(args.copy_calibration)
     if args.large_only:
         print(json.dumps({"large_batch": large_batch()}))
+        return 0
+    if args.subapertures_only:
+        print(json.dumps({"subapertures": subapertures()}))
         return 0
 
     import torch

@@ -169,13 +169,15 @@ ISAR_API int isar_session_dump(unsigned mask, const char* dir);
  * phases bit for bit. */
 enum isar_autofocus_mode { ISAR_AF_AUTO = 0, ISAR_AF_FUSED = 1, ISAR_AF_SPLIT = 2 };
 ISAR_API int isar_set_autofocus_mode(isar_context* ctx, int mode);
+ISAR_API int isar_get_autofocus_mode(isar_context* ctx);
 
 /* Rows of the range-alignment update handled per workgroup (2, 4, 8 or 16; default 8). The
  * choice changes the summation order of the group energies, not the algorithm. */
 ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows);
 ISAR_API int isar_get_ra_chunk(isar_context* ctx);
 /* Autotuner: times every range-alignment chunk size on these frames (`repeats` runs each after
- * one warm-up, on `stream`), keeps the fastest for the context and returns it in *chosen.
+ * one warm-up, on `stream`), keeps the fastest for the context and returns it in *chosen; then
+ * times both autofocus forms with that chunk size and keeps the faster (isar_get_autofocus_mode).
  * Synchronous; the image buffer receives the last run's images. */
 ISAR_API int isar_autotune(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity, float* d_img,
                            const isar_options* opt, int repeats, int32_t* chosen, void* stream);

@@ -316,30 +316,53 @@ int Context::autotune(int frames, const void* echo, const double* velocity, cons
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
         return fail(ISAR_ERR_DEVICE, "autotune: events");
     int rc = ISAR_OK;
+    // one warm-up and `repeats` timed runs of the current configuration; negative on failure
+    auto time_runs = [&]() -> float {
+        rc = run_frames(frames, echo, velocity, frame_row, nullptr, img, frame_img, opt, s);
+        if (rc) return -1.0f;
+        (void)hipEventRecord(a, s);
+        for (int r = 0; r < repeats && rc == ISAR_OK; ++r)
+            rc = run_frames(frames, echo, velocity, frame_row, nullptr, img, frame_img, opt, s);
+        (void)hipEventRecord(b, s);
+        if (rc) return -1.0f;
+        if (hipEventSynchronize(b) != hipSuccess) {
+            rc = fail(ISAR_ERR_DEVICE, "autotune: timing");
+            return -1.0f;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        return ms;
+    };
+    // range-alignment chunk rows
     float best_ms = 0.0f;
     int best = ra_chunk;
     for (int c : {2, 4, 8, 16}) {
         if (c > p.echo_num / 2) continue;
         ra_chunk = c;
-        rc = run_frames(frames, echo, velocity, frame_row, nullptr, img, frame_img, opt, s);   // warm-up
-        if (rc) break;
-        (void)hipEventRecord(a, s);
-        for (int r = 0; r < repeats && rc == ISAR_OK; ++r)
-            rc = run_frames(frames, echo, velocity, frame_row, nullptr, img, frame_img, opt, s);
-        (void)hipEventRecord(b, s);
-        if (rc) break;
-        if (hipEventSynchronize(b) != hipSuccess) {
-            rc = fail(ISAR_ERR_DEVICE, "autotune: timing");
-            break;
-        }
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, a, b);
+        const float ms = time_runs();
+        if (ms < 0.0f) break;
         if (best_ms == 0.0f || ms < best_ms) {
             best_ms = ms;
             best = c;
         }
     }
     ra_chunk = best;
+    // autofocus form, with that chunk size
+    if (rc == ISAR_OK) {
+        float best_af = 0.0f;
+        int best_mode = af_mode;
+        for (int mode : {ISAR_AF_FUSED, ISAR_AF_SPLIT}) {
+            af_mode = mode;
+            const float ms = time_runs();
+            if (ms < 0.0f) break;
+            if (best_af == 0.0f || ms < best_af) {
+                best_af = ms;
+                best_mode = mode;
+            }
+        }
+        af_mode = best_mode;
+        tuned_ms = (best_af > 0.0f ? best_af : best_ms) / static_cast<float>(repeats);
+    }
     if (chosen) *chosen = best;
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
@@ -536,6 +559,8 @@ ISAR_API int isar_set_autofocus_mode(isar_context* ctx, int mode) {
     ctx->af_mode = mode;
     return ISAR_OK;
 }
+
+ISAR_API int isar_get_autofocus_mode(isar_context* ctx) { return ctx ? ctx->af_mode : -1; }
 
 ISAR_API int isar_set_hrrp_output(isar_context* ctx, void* d_hrrp) {
     if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");

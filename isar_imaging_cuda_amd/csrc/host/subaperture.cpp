@@ -1,7 +1,9 @@
 // subaperture.cpp — sub-aperture plans behind isar_subaperture.h: windows grouped by pulse
 // count, one imaging context per group reading straight out of the dwell, the launches of all
-// groups captured into one HIP graph, and a per-group autotuner for the range-alignment chunk.
+// groups captured into one HIP graph, and a per-group autotuner for the range-alignment chunk
+// and the autofocus form.
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -18,7 +20,17 @@ struct SaGroup {
     Context ctx;
     long long* d_rows = nullptr;        // first dwell row of each window
     long long* d_img = nullptr;         // image offset of each window
+    double cost = 0.0;                  // expected time: a model at creation, the autotuner's measure after
+    int lane = 0;                       // branch stream the group is captured on
 };
+
+// Hardware queues a process gets (HIP's default, or GPU_MAX_HW_QUEUES when set): streams beyond
+// that share queues, and the groups on a shared queue run one after the other.
+inline int hw_queues() {
+    const char* v = std::getenv("GPU_MAX_HW_QUEUES");
+    const int n = v ? std::atoi(v) : 0;
+    return n > 0 ? n : 4;
+}
 
 struct SaPlan {
     isar_radar_params p{};
@@ -28,7 +40,7 @@ struct SaPlan {
     size_t img_floats = 0;
     std::vector<SaGroup*> groups;
     hipStream_t stream = nullptr;       // capture and replay stream (default priority)
-    std::vector<hipStream_t> branch;    // one per group: independent graph branches
+    std::vector<hipStream_t> branch;    // independent graph branches, at most one per hardware queue
     std::vector<hipEvent_t> joins;
     hipEvent_t ev_in = nullptr, ev_out = nullptr, ev_fork = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -60,17 +72,32 @@ struct SaPlan {
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    // During capture on `stream`: every group on its own branch stream, forked from and joined
-    // back into the capture stream, so the graph holds one independent branch per group.
+    // Groups onto branches, longest expected time first onto the least loaded branch, so that
+    // the branch that finishes last carries as little as possible.
+    void assign_lanes() {
+        std::vector<SaGroup*> order(groups);
+        std::stable_sort(order.begin(), order.end(), [](const SaGroup* a, const SaGroup* b) { return a->cost > b->cost; });
+        std::vector<double> load(branch.size(), 0.0);
+        for (SaGroup* g : order) {
+            const size_t b = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+            g->lane = static_cast<int>(b);
+            load[b] += g->cost;
+        }
+    }
+
+    // During capture on `stream`: each branch stream is forked from the capture stream, runs its
+    // groups in turn, and is joined back, so the graph holds one independent branch per stream.
     int capture_branches(const void* dwell_ptr, const double* vel, float* img, const isar_options& opt) {
         if (hipEventRecord(ev_fork, stream) != hipSuccess) return fail(ISAR_ERR_DEVICE, "fork");
-        for (size_t gi = 0; gi < groups.size(); ++gi) {
-            SaGroup* g = groups[gi];
-            if (hipStreamWaitEvent(branch[gi], ev_fork, 0) != hipSuccess) return fail(ISAR_ERR_DEVICE, "fork wait");
-            const int rc = g->ctx.run_frames(static_cast<int>(g->windows.size()), dwell_ptr, vel, g->d_rows, nullptr,
-                                             img, g->d_img, opt, branch[gi]);
-            if (rc) return rc;
-            if (hipEventRecord(joins[gi], branch[gi]) != hipSuccess || hipStreamWaitEvent(stream, joins[gi], 0) != hipSuccess)
+        for (size_t b = 0; b < branch.size(); ++b) {
+            if (hipStreamWaitEvent(branch[b], ev_fork, 0) != hipSuccess) return fail(ISAR_ERR_DEVICE, "fork wait");
+            for (SaGroup* g : groups) {
+                if (g->lane != static_cast<int>(b)) continue;
+                const int rc = g->ctx.run_frames(static_cast<int>(g->windows.size()), dwell_ptr, vel, g->d_rows, nullptr,
+                                                 img, g->d_img, opt, branch[b]);
+                if (rc) return rc;
+            }
+            if (hipEventRecord(joins[b], branch[b]) != hipSuccess || hipStreamWaitEvent(stream, joins[b], 0) != hipSuccess)
                 return fail(ISAR_ERR_DEVICE, "join");
         }
         return ISAR_OK;
@@ -121,6 +148,9 @@ ISAR_API int isar_sa_plan_create(const isar_radar_params* p, int dwell_pulses, i
         plan->groups.push_back(g);
         g->M = M;
         g->windows = wins;
+        // cost model before any measurement: a per-iteration autofocus latency that grows with
+        // the pulse count plus range-alignment work per window
+        g->cost = static_cast<double>(M) * (static_cast<double>(wins.size()) + 16.0);
         isar_radar_params gp = *p;
         gp.echo_num = M;
         int rc = g->ctx.init(gp, static_cast<int>(wins.size()));
@@ -145,7 +175,8 @@ ISAR_API int isar_sa_plan_create(const isar_radar_params* p, int dwell_pulses, i
               hipEventCreateWithFlags(&plan->ev_in, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&plan->ev_out, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&plan->ev_fork, hipEventDisableTiming) == hipSuccess;
-    for (size_t gi = 0; ok && gi < plan->groups.size(); ++gi) {
+    const size_t nbranch = std::min(plan->groups.size(), static_cast<size_t>(isar::hw_queues()));
+    for (size_t bi = 0; ok && bi < nbranch; ++bi) {
         hipStream_t b = nullptr;
         hipEvent_t e = nullptr;
         ok = hipStreamCreateWithFlags(&b, hipStreamNonBlocking) == hipSuccess;
@@ -157,6 +188,7 @@ ISAR_API int isar_sa_plan_create(const isar_radar_params* p, int dwell_pulses, i
         delete plan;
         return fail(ISAR_ERR_DEVICE, "isar_sa_plan_create: streams");
     }
+    plan->assign_lanes();
     *out = plan;
     return ISAR_OK;
 }
@@ -234,7 +266,9 @@ ISAR_API int isar_sa_plan_autotune(isar_sa_plan* plan, const void* d_dwell, cons
                                        g->d_img, *opt, plan->stream, repeats, &c);
         if (rc) return rc;
         if (chosen) chosen[gi] = c;
+        g->cost = g->ctx.tuned_ms;
     }
+    plan->assign_lanes();
     return ISAR_OK;
 }
 
