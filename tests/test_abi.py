@@ -25,7 +25,8 @@ def _declared(header: str) -> list[str]:
     return sorted(set(names))
 
 
-@pytest.mark.parametrize("header", ["isar_imaging_export.h", "isar_imaging_ext.h", "isar_learned.h", "isar_synth.h"])
+@pytest.mark.parametrize("header", ["isar_imaging_export.h", "isar_imaging_ext.h", "isar_learned.h", "isar_synth.h",
+                                    "isar_subaperture.h"])
 def test_library_exports_every_declared_entry_point(header):
     names = _declared(header)
     assert names, header
@@ -49,6 +50,37 @@ def test_invalid_sizes_return_error_codes():
         assert "rc=-1" in str(e.value)
     msg = isar.lib().isar_last_error()
     assert msg and len(msg) > 0
+
+
+def test_new_entry_points_reject_bad_arguments_without_a_device():
+    """Argument checks that run before any HIP call (error codes of include/isar_imaging_ext.h)."""
+    L = isar.lib()
+    vp = C.c_void_p
+    dummy = vp(16)                                 # never dereferenced: the checks come first
+    L.isar_pulse_compress.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, vp]
+    p = isar.RadarParams(128, 4096, 20_000_000, 10_000_000_000, 40e-6, 32_000_000).c()
+    assert L.isar_pulse_compress(C.byref(p), 1, 100, dummy, dummy, dummy, None, None) == -1
+    assert "unsupported transform size 128" in L.isar_last_error().decode()
+    p_long = isar.RadarParams(128, 4096, 20_000_000, 10_000_000_000, 1e-3, 32_000_000).c()   # Fs Tp > NFFT
+    assert L.isar_pulse_compress(C.byref(p_long), 1, 6000, dummy, dummy, dummy, None, None) == -1
+    assert "reference chirp" in L.isar_last_error().decode()
+    assert L.isar_pulse_compress(C.byref(p), 0, 6000, dummy, dummy, dummy, None, None) == -1
+    L.isar_set_autofocus_mode.argtypes = [vp, C.c_int]
+    assert L.isar_set_autofocus_mode(None, 1) == -1
+    L.isar_set_hrrp_output.argtypes = [vp, vp]
+    assert L.isar_set_hrrp_output(None, dummy) == -1
+    L.isar_stage_bytes.argtypes = [vp, C.c_int]
+    L.isar_stage_bytes.restype = C.c_size_t
+    assert L.isar_stage_bytes(None, 0) == 0
+    L.isar_stage_read.argtypes = [vp, C.c_int, C.c_int, vp]
+    assert L.isar_stage_read(None, 0, 0, dummy) == -1
+    L.isar_select_phase.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
+    assert L.isar_select_phase(0, 256, dummy, dummy, dummy, dummy, dummy, None) == -1
+    # the drop-in session's extras before imagingMemInit: out of order
+    L.isar_session_hrrp.argtypes = [vp]
+    assert L.isar_session_hrrp(dummy) == -4
+    L.isar_session_dump.argtypes = [C.c_uint, C.c_char_p]
+    assert L.isar_session_dump(0x7F, b"/tmp") == -4
 
 
 def test_workspace_bytes_scales_with_frames():
