@@ -163,11 +163,13 @@ ISAR_API int isar_stage_dump(isar_context* ctx, int frame, unsigned mask, const 
 ISAR_API int isar_session_dump(unsigned mask, const char* dir);
 
 /* Form of the fast-entropy autofocus. FUSED runs each frame's 120 iterations in one workgroup,
- * which fills the GPU when a batch has hundreds of frames; SPLIT spreads each frame's selected
- * range bins over several workgroups with two launches per iteration, for small batches and
- * single frames. AUTO (default) splits batches of at most 32 frames. Both forms give the same
- * phases bit for bit. */
-enum isar_autofocus_mode { ISAR_AF_AUTO = 0, ISAR_AF_FUSED = 1, ISAR_AF_SPLIT = 2 };
+ * which fills the GPU when a batch has hundreds of frames. SPLIT spreads each frame's selected
+ * range bins over several workgroups, for small batches and single frames: one launch per
+ * iteration, every workgroup forming the current phase from the previous iteration's sums.
+ * SPLIT_PHASE does the same with a separate phase launch per iteration, which repeats no work
+ * and suits GPUs shared with other streams. AUTO (default) uses SPLIT for batches of at most 32
+ * frames and FUSED above. All forms give the same phases bit for bit. */
+enum isar_autofocus_mode { ISAR_AF_AUTO = 0, ISAR_AF_FUSED = 1, ISAR_AF_SPLIT = 2, ISAR_AF_SPLIT_PHASE = 3 };
 ISAR_API int isar_set_autofocus_mode(isar_context* ctx, int mode);
 ISAR_API int isar_get_autofocus_mode(isar_context* ctx);
 
@@ -177,7 +179,7 @@ ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows);
 ISAR_API int isar_get_ra_chunk(isar_context* ctx);
 /* Autotuner: times every range-alignment chunk size on these frames (`repeats` runs each after
  * one warm-up, on `stream`), keeps the fastest for the context and returns it in *chosen; then
- * times both autofocus forms with that chunk size and keeps the faster (isar_get_autofocus_mode).
+ * times the autofocus forms with that chunk size and keeps the fastest (isar_get_autofocus_mode).
  * Synchronous; the image buffer receives the last run's images. */
 ISAR_API int isar_autotune(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity, float* d_img,
                            const isar_options* opt, int repeats, int32_t* chosen, void* stream);

@@ -49,7 +49,7 @@ class _Trace(C.Structure):
 
 # kernel families (enum isar_stage in include/isar_imaging_ext.h)
 # isar_autofocus_mode
-AF_AUTO, AF_FUSED, AF_SPLIT = 0, 1, 2
+AF_AUTO, AF_FUSED, AF_SPLIT, AF_SPLIT_PHASE = 0, 1, 2, 3
 
 # arrays of isar_stage_read (the reference's DATA_WRITE_BACK_* dumps): name -> (id, complex?, columns)
 DUMP_STAGES = {"dataW": 0, "hpc": 1, "hrrp": 2, "ra": 3, "pc": 4, "mtrc": 5, "final": 6}
@@ -259,8 +259,9 @@ class Imager:
         _check(L.isar_stage_dump(self._h, int(frame), mask, str(directory).encode()), "isar_stage_dump")
 
     def set_autofocus_mode(self, mode: int) -> None:
-        """AF_AUTO (split batches of at most 32 frames), AF_FUSED (one workgroup per frame) or
-        AF_SPLIT (several workgroups per frame); the phases are the same bit for bit."""
+        """AF_AUTO (split batches of at most 32 frames), AF_FUSED (one workgroup per frame),
+        AF_SPLIT (several workgroups per frame, one launch per iteration) or AF_SPLIT_PHASE (the
+        same with a phase launch per iteration); the phases are the same bit for bit."""
         L = lib()
         L.isar_set_autofocus_mode.argtypes = [C.c_void_p, C.c_int]
         _check(L.isar_set_autofocus_mode(self._h, int(mode)), "isar_set_autofocus_mode")

@@ -96,7 +96,7 @@ size_t workspace_bytes(const isar_radar_params& p, int K) {
     per += M * sizeof(float2) * 3;              // rowdop, phi0, phi
     per += nbands * W * sizeof(float) + 3 * W * sizeof(float) + W * sizeof(int32_t) + W * sizeof(float2);
     per += W * sizeof(float2);                  // delta
-    per += (W / 8) * M * sizeof(float2);        // af_part
+    per += 2 * (W / 8) * M * sizeof(float2);    // af_part (two iterations' halves)
     // shared: chirp-z filter spectra, input/output chirps, range window
     return per * K + W * F * sizeof(float2) + 2 * W * M * sizeof(float2) + N * sizeof(float);
 }
@@ -195,7 +195,7 @@ int Context::init(const isar_radar_params& params, int frames) {
     float2* czt_pre = static_cast<float2*>(alloc(W * M * sizeof(float2)));
     float2* czt_post = static_cast<float2*>(alloc(W * M * sizeof(float2)));
     J.delta = static_cast<float2*>(alloc(K * W * sizeof(float2)));
-    J.af_part = static_cast<float2*>(alloc(K * (W / 8) * M * sizeof(float2)));
+    J.af_part = static_cast<float2*>(alloc(2 * K * (W / 8) * M * sizeof(float2)));
     float* win_range = static_cast<float*>(alloc(N * sizeof(float)));
     if (!ok) return fail(ISAR_ERR_MEMORY, "device allocation failed for " + std::to_string(frames) + " frames");
     // constants (high_speed_compensation.cu:16-18, mtrc.cu:10-12, 47)
@@ -284,9 +284,10 @@ int Context::run_frames(int frames, const void* echo, const double* velocity, co
         probed(ISAR_STAGE_PA_COLFFT, s, [&] { launch_pa_colfft(J, s); });
         probed(ISAR_STAGE_PA_SELECT, s, [&] { launch_pa_select(J, s); });
     }
-    const bool af_split = af_mode == ISAR_AF_SPLIT || (af_mode == ISAR_AF_AUTO && frames <= kAfSplitFrames);
+    const bool af_split = af_mode == ISAR_AF_SPLIT || af_mode == ISAR_AF_SPLIT_PHASE ||
+                          (af_mode == ISAR_AF_AUTO && frames <= kAfSplitFrames);
     probed(ISAR_STAGE_AUTOFOCUS, s, [&] {
-        if (af_split) launch_autofocus_split(J, s);
+        if (af_split) launch_autofocus_split(J, af_mode != ISAR_AF_SPLIT_PHASE, s);
         else launch_autofocus(J, s);
     });
     if (mode == ISAR_RUN_FRONT || (mode == ISAR_RUN_BACK && !img)) {
@@ -351,10 +352,11 @@ int Context::autotune(int frames, const void* echo, const double* velocity, cons
     if (rc == ISAR_OK) {
         float best_af = 0.0f;
         int best_mode = af_mode;
-        for (int mode : {ISAR_AF_FUSED, ISAR_AF_SPLIT}) {
+        for (int mode : {ISAR_AF_FUSED, ISAR_AF_SPLIT, ISAR_AF_SPLIT_PHASE}) {
             af_mode = mode;
             const float ms = time_runs();
             if (ms < 0.0f) break;
+            mode_ms[mode] = ms / static_cast<float>(repeats);
             if (best_af == 0.0f || ms < best_af) {
                 best_af = ms;
                 best_mode = mode;
@@ -554,8 +556,9 @@ ISAR_API int isar_get_ra_chunk(isar_context* ctx) { return ctx ? ctx->ra_chunk :
 
 ISAR_API int isar_set_autofocus_mode(isar_context* ctx, int mode) {
     if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");
-    if (mode != ISAR_AF_AUTO && mode != ISAR_AF_FUSED && mode != ISAR_AF_SPLIT)
-        return isar::fail(ISAR_ERR_INVALID, "autofocus mode must be 0 (auto), 1 (one workgroup per frame) or 2 (split)");
+    if (mode != ISAR_AF_AUTO && mode != ISAR_AF_FUSED && mode != ISAR_AF_SPLIT && mode != ISAR_AF_SPLIT_PHASE)
+        return isar::fail(ISAR_ERR_INVALID,
+                          "autofocus mode must be 0 (auto), 1 (one workgroup per frame), 2 (split) or 3 (split, phase launches)");
     ctx->af_mode = mode;
     return ISAR_OK;
 }

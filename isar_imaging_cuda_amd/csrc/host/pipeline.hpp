@@ -61,6 +61,7 @@ struct Context {
     float2* hrrp_out = nullptr;           // max_frames x M x N profiles written by if_hrrp runs, or null
     int af_mode = ISAR_AF_AUTO;           // autofocus form: auto, one workgroup per frame, split
     float tuned_ms = 0.0f;                // time of one run in the configuration the autotuner kept
+    float mode_ms[4] = {0, 0, 0, 0};      // time of one run per autofocus form (index: isar_autofocus_mode)
     int ra_chunk = kRaChunkRowsDefault;   // tunable: rows per workgroup in the alignment update
     // times every range-alignment chunk size on these frames and keeps the fastest (synchronous)
     int autotune(int frames, const void* echo, const double* velocity, const long long* frame_row, float* img,

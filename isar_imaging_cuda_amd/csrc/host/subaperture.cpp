@@ -269,7 +269,57 @@ ISAR_API int isar_sa_plan_autotune(isar_sa_plan* plan, const void* d_dwell, cons
         g->cost = g->ctx.tuned_ms;
     }
     plan->assign_lanes();
-    return ISAR_OK;
+    // The groups share the GPU when the graph runs, so the autofocus form each group won alone
+    // is checked against one form for every group, timing whole graph replays; each candidate
+    // places the groups on branches by their measured times in that form.
+    hipStream_t ts = nullptr;
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipStreamCreateWithFlags(&ts, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&a) != hipSuccess ||
+        hipEventCreate(&b) != hipSuccess)
+        return fail(ISAR_ERR_DEVICE, "isar_sa_plan_autotune: timing objects");
+    // candidates: each group's fastest form alone among a subset of the forms, and one form for all
+    const std::vector<std::vector<int>> subsets = {
+        {ISAR_AF_FUSED, ISAR_AF_SPLIT, ISAR_AF_SPLIT_PHASE}, {ISAR_AF_FUSED, ISAR_AF_SPLIT_PHASE},
+        {ISAR_AF_FUSED, ISAR_AF_SPLIT}, {ISAR_AF_FUSED}, {ISAR_AF_SPLIT}, {ISAR_AF_SPLIT_PHASE}};
+    auto pick = [&](const std::vector<int>& forms) {
+        for (isar::SaGroup* g : plan->groups) {
+            int m = forms[0];
+            for (int f : forms)
+                if (g->ctx.mode_ms[f] > 0.0f && g->ctx.mode_ms[f] < g->ctx.mode_ms[m]) m = f;
+            g->ctx.af_mode = m;
+            g->cost = g->ctx.mode_ms[m];
+        }
+        plan->assign_lanes();
+    };
+    int rc = ISAR_OK;
+    float best_ms = 0.0f;
+    size_t best = 0;
+    for (size_t c = 0; c < subsets.size(); ++c) {
+        pick(subsets[c]);
+        plan->drop_graph();
+        rc = isar_sa_plan_run(plan, d_dwell, d_velocity, d_img, opt, 1, ts);   // capture
+        if (rc) break;
+        (void)hipEventRecord(a, ts);
+        for (int r = 0; r < repeats && rc == ISAR_OK; ++r) rc = isar_sa_plan_run(plan, d_dwell, d_velocity, d_img, opt, 1, ts);
+        (void)hipEventRecord(b, ts);
+        if (rc) break;
+        if (hipEventSynchronize(b) != hipSuccess) {
+            rc = fail(ISAR_ERR_DEVICE, "isar_sa_plan_autotune: timing");
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        if (best_ms == 0.0f || ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    pick(subsets[best]);
+    plan->drop_graph();
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipStreamDestroy(ts);
+    return rc;
 }
 
 }  // extern "C"

@@ -298,26 +298,60 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
 
 // The same fixed-point update with a frame spread over several workgroups, for batches too small
 // to fill the GPU with one workgroup per frame. k_autofocus with no iterations gathers the
-// selected bins and forms the starting phase; then each iteration is k_af_cols, where workgroup
-// (j, f) transforms sub-chunk j of frame f's selected bins and writes its per-pulse sums of
-// conj(S) IFFT(G |G|) / M, and k_af_phase, which adds the sub-chunk sums in order and normalises.
-// The transforms and the summation order are those of k_autofocus, so the phases agree bit for bit.
+// selected bins and forms the starting phase; then iteration `it` is a launch of k_af_cols:
+// workgroup (j, f) takes the frame's current phase, transforms sub-chunk j of the selected bins
+// and writes its per-pulse sums of conj(S) IFFT(G |G|) / M into the iteration's half of af_part.
+// The current phase is either formed by every workgroup alike from the previous iteration's sums
+// (FOLD: one launch per iteration) or read from J.phi, which k_af_phase writes after each
+// iteration (two launches per iteration, no repeated work). The transforms and the summation
+// order are those of k_autofocus, so the phases agree bit for bit.
 template <int M>
-__global__ __launch_bounds__(AfCfg<M>::SUB * AfCfg<M>::TPF) void k_af_cols(Job J) {
+__device__ __forceinline__ float2 af_phase_from(const float2* part, int nsub, int m) {
+    float2 acc = make_float2(0.0f, 0.0f);
+    int q = 0;
+    for (; q + 8 <= nsub; q += 8) {   // eight loads in flight, added in order
+        float2 v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = part[static_cast<size_t>(q + i) * M + m];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc = cadd(acc, v[i]);
+    }
+    for (; q < nsub; ++q) acc = cadd(acc, part[static_cast<size_t>(q) * M + m]);
+    const float a = cabsf_(acc);
+    return a > 0.0f ? cscale(acc, 1.0f / a) : make_float2(0.0f, 0.0f);
+}
+
+// the half of af_part that iteration `it` writes; frame f's sub-chunk sums in it
+template <int M>
+__device__ __forceinline__ float2* af_part_of(const Job& J, int f, int it) {
+    const size_t per_frame = static_cast<size_t>(J.W / AfCfg<M>::SUB) * M;
+    return J.af_part + (static_cast<size_t>(it & 1) * J.K + f) * per_frame;
+}
+
+template <int M, bool FOLD>
+__global__ __launch_bounds__(AfCfg<M>::SUB * AfCfg<M>::TPF) void k_af_cols(Job J, int it) {
     using A = AfCfg<M>;
     using C = ColCfg<M>;
     constexpr int E = A::E, TPF = A::TPF, SUB = A::SUB, NTS = SUB * TPF;
-    HIP_DYNAMIC_SHARED(float2, bufs);   // SUB * lds_padded_size(M)
+    HIP_DYNAMIC_SHARED(float2, bufs);   // SUB * lds_padded_size(M), then (FOLD) the phase (M)
     const int j = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
     const int U = J.dec[f].unit2_num;
+    const int nsub = (U + SUB - 1) / SUB;
+    if (j >= nsub) return;   // the whole workgroup leaves before any barrier
     const int c0 = j * SUB;
-    if (c0 >= U) return;   // the whole workgroup leaves before any barrier
     const int W = J.W;
+    const float2* phi = J.phi + static_cast<size_t>(f) * M;
+    if (FOLD && it > 0) {
+        float2* formed = bufs + SUB * lds_padded_size(M);
+        const float2* prev = af_part_of<M>(J, f, it - 1);
+        for (int m = tid; m < M; m += NTS) formed[m] = af_phase_from<M>(prev, nsub, m);
+        __syncthreads();
+        phi = formed;
+    }
     const int g = tid / TPF, lane = tid % TPF;
     const int u = c0 + g;
     const bool valid = u < U;
     const float2* col = J.su + static_cast<size_t>(f) * W * M + static_cast<size_t>(valid ? u : 0) * M;
-    const float2* phi = J.phi + static_cast<size_t>(f) * M;
     float2* buf = bufs + g * lds_padded_size(M);
     const float invM = 1.0f / static_cast<float>(M);
     const FftTwiddles<M, E> tw = fft_twiddles<M, E>(lane);
@@ -339,32 +373,29 @@ __global__ __launch_bounds__(AfCfg<M>::SUB * AfCfg<M>::TPF) void k_af_cols(Job J
     }
     __syncthreads();
     const int ncol = min(SUB, U - c0);
-    float2* P = J.af_part + (static_cast<size_t>(f) * (W / SUB) + j) * M;
+    float2* part = af_part_of<M>(J, f, it) + static_cast<size_t>(j) * M;
     for (int m = tid; m < M; m += NTS) {
         float2 t = make_float2(0.0f, 0.0f);
         for (int k = 0; k < ncol; ++k) t = cadd(t, bufs[k * lds_padded_size(M) + m]);
-        P[m] = t;
+        part[m] = t;
     }
 }
 
 template <int M>
-__global__ __launch_bounds__(256) void k_af_phase(Job J) {
+__global__ __launch_bounds__(256) void k_af_phase(Job J, int it) {
     using A = AfCfg<M>;
     const int f = blockIdx.y, m = blockIdx.x * 256 + threadIdx.x;
     if (m >= M) return;
-    const int U = J.dec[f].unit2_num;
-    const int nsub = (U + A::SUB - 1) / A::SUB;
-    const float2* P = J.af_part + static_cast<size_t>(f) * (J.W / A::SUB) * M + m;
-    float2 acc = make_float2(0.0f, 0.0f);
-    for (int j = 0; j < nsub; ++j) acc = cadd(acc, P[static_cast<size_t>(j) * M]);
-    const float a = cabsf_(acc);
-    J.phi[static_cast<size_t>(f) * M + m] = a > 0.0f ? cscale(acc, 1.0f / a) : make_float2(0.0f, 0.0f);
+    const int nsub = (J.dec[f].unit2_num + A::SUB - 1) / A::SUB;
+    J.phi[static_cast<size_t>(f) * M + m] = af_phase_from<M>(af_part_of<M>(J, f, it), nsub, m);
 }
 
 // ------------------------------------------------------------------------------------------
 namespace {
 template <int M>
-size_t af_cols_lds() { return static_cast<size_t>(AfCfg<M>::SUB) * lds_padded_size(M) * sizeof(float2); }
+size_t af_cols_lds(bool fold) {
+    return (static_cast<size_t>(AfCfg<M>::SUB) * lds_padded_size(M) + (fold ? M : 0)) * sizeof(float2);
+}
 template <int M>
 size_t colfft_lds() {
     using C = ColCfg<M>;
@@ -408,7 +439,8 @@ hipError_t configure_pa_kernels() {
 #define ISAR_SET_PA(MM)                                                          \
     set(reinterpret_cast<const void*>(&k_pa_colfft<MM>), colfft_lds<MM>());      \
     set(reinterpret_cast<const void*>(&k_autofocus<MM>), autofocus_lds<MM>());   \
-    set(reinterpret_cast<const void*>(&k_af_cols<MM>), af_cols_lds<MM>());
+    set(reinterpret_cast<const void*>(&k_af_cols<MM, true>), af_cols_lds<MM>(true));   \
+    set(reinterpret_cast<const void*>(&k_af_cols<MM, false>), af_cols_lds<MM>(false));
     ISAR_SET_PA(64) ISAR_SET_PA(128) ISAR_SET_PA(192) ISAR_SET_PA(256) ISAR_SET_PA(384) ISAR_SET_PA(512)
     ISAR_SET_PA(1024)
 #undef ISAR_SET_PA
@@ -426,17 +458,30 @@ void launch_pa_select(const Job& J, hipStream_t s) {
 void launch_autofocus(const Job& J, hipStream_t s) {
     ISAR_PULSE_SWITCH(J.M, ISAR_LAUNCH(k_autofocus<MM>, dim3(J.K), dim3(AfCfg<MM>::NT), autofocus_lds<MM>(), s, J));
 }
-void launch_autofocus_split(const Job& J, hipStream_t s) {
+namespace {
+template <int MM>
+void autofocus_split(const Job& J, bool fold, hipStream_t s) {
     Job J0 = J;
     J0.iterations = 0;
-    ISAR_PULSE_SWITCH(J.M, {
-        ISAR_LAUNCH(k_autofocus<MM>, dim3(J.K), dim3(AfCfg<MM>::NT), autofocus_lds<MM>(), s, J0);
-        constexpr int SUB = AfCfg<MM>::SUB;
-        for (int it = 0; it < J.iterations; ++it) {
-            ISAR_LAUNCH(k_af_cols<MM>, dim3(J.W / SUB, J.K), dim3(SUB * AfCfg<MM>::TPF), af_cols_lds<MM>(), s, J);
-            ISAR_LAUNCH(k_af_phase<MM>, dim3((MM + 255) / 256, J.K), dim3(256), 0, s, J);
+    ISAR_LAUNCH(k_autofocus<MM>, dim3(J.K), dim3(AfCfg<MM>::NT), autofocus_lds<MM>(), s, J0);
+    constexpr int SUB = AfCfg<MM>::SUB;
+    const dim3 cols_grid(J.W / SUB, J.K);
+    const dim3 cols_block(SUB * AfCfg<MM>::TPF);
+    const dim3 phase_grid((MM + 255) / 256, J.K);
+    for (int it = 0; it < J.iterations; ++it) {
+        if (fold) {
+            ISAR_LAUNCH((k_af_cols<MM, true>), cols_grid, cols_block, af_cols_lds<MM>(true), s, J, it);
+        } else {
+            ISAR_LAUNCH((k_af_cols<MM, false>), cols_grid, cols_block, af_cols_lds<MM>(false), s, J, it);
+            ISAR_LAUNCH(k_af_phase<MM>, phase_grid, dim3(256), 0, s, J, it);
         }
-    });
+    }
+    if (fold && J.iterations > 0) ISAR_LAUNCH(k_af_phase<MM>, phase_grid, dim3(256), 0, s, J, J.iterations - 1);
+}
+}  // namespace
+
+void launch_autofocus_split(const Job& J, bool fold, hipStream_t s) {
+    ISAR_PULSE_SWITCH(J.M, autofocus_split<MM>(J, fold, s));
 }
 
 }  // namespace isar

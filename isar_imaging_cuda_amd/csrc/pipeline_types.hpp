@@ -76,7 +76,7 @@ struct Job {
     float2* stm;                  // K x M x W (MTRC range-frequency rows)
     float2* czt;                  // K x M x W (chirp-z output)
     float2* colsum;               // K x W
-    float2* af_part;              // K x (W/8) x M: per-pulse sums per sub-chunk of selected bins (split autofocus)
+    float2* af_part;              // 2 x K x (W/8) x M: per-pulse sums per sub-chunk of selected bins, by iteration parity (split autofocus)
     const float2* czt_table;      // W x F (FFT of the Bluestein chirp per range bin)
     const float2* czt_pre;        // W x M: exp(j pi a_n m (M - m) / M), the input chirp
     const float2* czt_post;       // W x M: exp(-j pi a_n k^2 / M) / F, the output chirp

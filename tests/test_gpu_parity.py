@@ -103,16 +103,18 @@ def test_autofocus_forms_agree(M, N):
     print(f"[case] autofocus forms M={M} N={N}", flush=True)
     params = isar.RadarParams(M, N, radar["band_width"], radar["fc"], radar["Tp"], radar["Fs"])
     out = {}
-    for mode in (isar.AF_FUSED, isar.AF_SPLIT):
+    for mode in (isar.AF_FUSED, isar.AF_SPLIT, isar.AF_SPLIT_PHASE):
         with isar.Imager(params, max_frames=2) as im:
             im.set_autofocus_mode(mode)
             img = im.run_host(echo, vel, isar.Options())
             out[mode] = (img, [im.trace(k) for k in range(2)])
-    (img_f, tf), (img_s, ts) = out[isar.AF_FUSED], out[isar.AF_SPLIT]
-    for k in range(2):
-        assert tf[k].unit2_num == ts[k].unit2_num
-        assert np.array_equal(tf[k].phi, ts[k].phi), f"frame {k}: phases differ"
-    assert np.array_equal(img_f, img_s)
+    img_f, tf = out[isar.AF_FUSED]
+    for mode in (isar.AF_SPLIT, isar.AF_SPLIT_PHASE):
+        img_s, ts = out[mode]
+        for k in range(2):
+            assert tf[k].unit2_num == ts[k].unit2_num
+            assert np.array_equal(tf[k].phi, ts[k].phi), f"mode {mode} frame {k}: phases differ"
+        assert np.array_equal(img_f, img_s), f"mode {mode}"
 
 
 def test_batch_matches_single_frames():
