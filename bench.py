@@ -76,8 +76,9 @@ def stage_bytes_per_frame(stage: str, M: int, N: int, W: int, U: float) -> float
     }[stage]
 
 
-def cpu_baseline(frames: int = 128) -> dict:
-    """FP64 oracle with OpenMP over frames on this host (bounded sample of the same workload)."""
+def cpu_baseline(frames: int = 64, runs: int = 3) -> dict:
+    """FP64 oracle with OpenMP over frames on this host (bounded sample of the same workload);
+    frames/s of the median of `runs` timed runs."""
     from isar_imaging_cuda_amd import synth
     from oracle import oracle as O
 
@@ -88,12 +89,15 @@ def cpu_baseline(frames: int = 128) -> dict:
     h = synth.radar_header(cfgs[0])
     p = O.params(M, N, W, h["band_width"], h["fc"], h["Tp"], h["Fs"])
     O.isar_batch(echo[:1], vel[:1], p)                      # warm-up (plan caches)
-    t0 = time.perf_counter()
-    _, threads = O.isar_batch(echo, vel, p)
-    dt = time.perf_counter() - t0
+    walls = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        _, threads = O.isar_batch(echo, vel, p)
+        walls.append(time.perf_counter() - t0)
+    dt = float(np.median(walls))
     return {"value": frames / dt, "unit": "frames/s", "cores": int(threads), "kind": "port",
-            "sample": f"{frames} distinct 256x4096 frames, FP64 oracle, one OpenMP thread per frame, "
-                      f"{dt:.1f} s wall"}
+            "sample": f"{frames} distinct 256x4096 frames, FP64 oracle, one OpenMP thread per frame (frame-level); "
+                      f"median of {runs} runs, {dt:.1f} s wall each"}
 
 
 def copy_calibration(mib: int) -> int:
@@ -269,7 +273,7 @@ def main() -> int:
     ap.add_argument("--probe", default="ra_upd", help="kernel family timed for the roofline figure")
     ap.add_argument("--probe-steps", type=int, default=3, help="extra steps timed per launch for the roofline")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-frames", type=int, default=128)
+    ap.add_argument("--cpu-frames", type=int, default=64)
     ap.add_argument("--ra-chunk", type=int, default=0, help="fix the range-alignment chunk size (no autotuning)")
     ap.add_argument("--autotune", action="store_true",
                     help="let the autotuner pick the range-alignment chunk size on this batch (default: 8 rows, "
@@ -338,17 +342,17 @@ def main() -> int:
     print(f"[bench] warm-up done: {K} frames/step, mean U = {U:.1f}", file=sys.stderr, flush=True)
 
     # timed region: exactly `steps` steps, nothing else on the stream
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
+    evs[0].record(stream)
+    for i in range(args.steps):
         step()
-    ev1.record(stream)
+        evs[i + 1].record(stream)
     torch.cuda.synchronize()
     wall_ms = (time.perf_counter() - t0) * 1e3
-    total_ms = ev0.elapsed_time(ev1)
+    total_ms = evs[0].elapsed_time(evs[-1])
+    step_ms = [evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps)]
     ms_per_step = total_ms / args.steps
     fps = K * args.steps / (total_ms / 1e3)
 
@@ -374,6 +378,7 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step_median": round(float(np.median(step_ms)), 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
