@@ -207,7 +207,8 @@ def large_batch(frames: int = 8, distinct: int = 2, steps: int = 3) -> dict:
 
 
 def single_frame(params, echo, vel, stream, runs: int = 20) -> dict:
-    """Latency of one frame alone (a batch of one, device-resident), the drop-in's unit of work."""
+    """Latency of one frame alone (a batch of one, device-resident), the drop-in's unit of work:
+    replayed as a HIP graph (graph mode, as the drop-in session runs) and launched directly."""
     import torch
 
     import isar_imaging_cuda_amd as isar
@@ -215,17 +216,22 @@ def single_frame(params, echo, vel, stream, runs: int = 20) -> dict:
     img1 = torch.empty((1, params.echo_num, W), dtype=torch.float32, device=echo.device)
     run = lambda: im.run_device(echo.data_ptr(), vel.data_ptr(), img1.data_ptr(), 1, isar.Options(),
                                 stream=stream.cuda_stream)
-    for _ in range(3):
-        run()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(runs):
-        run()
-    e1.record(stream)
-    torch.cuda.synchronize()
+    out = {}
+    for name, graph in (("ms_per_frame", True), ("ms_per_frame_direct", False)):
+        im.set_graph_mode(graph)
+        for _ in range(3):
+            run()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(runs):
+            run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out[name] = round(e0.elapsed_time(e1) / runs, 4)
     im.close()
-    return {"ms_per_frame": round(e0.elapsed_time(e1) / runs, 4), "runs": runs}
+    out["runs"] = runs
+    return out
 
 
 def host_transfer(echo, img, ms_per_step: float, stream) -> dict:

@@ -173,6 +173,12 @@ enum isar_autofocus_mode { ISAR_AF_AUTO = 0, ISAR_AF_FUSED = 1, ISAR_AF_SPLIT = 
 ISAR_API int isar_set_autofocus_mode(isar_context* ctx, int mode);
 ISAR_API int isar_get_autofocus_mode(isar_context* ctx);
 
+/* Graph mode (default off): a full run whose arguments (frame count, pointers, options, tuning)
+ * equal the previous full run's replays a HIP graph of its launches instead of launching them one
+ * by one; a run with new arguments is captured once first. Runs with probes enabled, and front or
+ * back runs, launch directly. The drop-in session turns it on: its buffers never change. */
+ISAR_API int isar_set_graph_mode(isar_context* ctx, int on);
+
 /* Rows of the range-alignment update handled per workgroup (2, 4, 8 or 16; default 8). The
  * choice changes the summation order of the group energies, not the algorithm. */
 ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows);

@@ -266,6 +266,12 @@ class Imager:
         L.isar_set_autofocus_mode.argtypes = [C.c_void_p, C.c_int]
         _check(L.isar_set_autofocus_mode(self._h, int(mode)), "isar_set_autofocus_mode")
 
+    def set_graph_mode(self, on: bool) -> None:
+        """Replay a captured HIP graph for full runs whose arguments repeat."""
+        L = lib()
+        L.isar_set_graph_mode.argtypes = [C.c_void_p, C.c_int]
+        _check(L.isar_set_graph_mode(self._h, int(bool(on))), "isar_set_graph_mode")
+
     def set_hrrp_output(self, hrrp_ptr: int | None) -> None:
         """Device buffer (max_frames x M x N complex64) for the range profiles that runs with
         if_hrrp write; None stops the output."""

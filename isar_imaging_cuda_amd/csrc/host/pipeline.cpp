@@ -102,6 +102,8 @@ size_t workspace_bytes(const isar_radar_params& p, int K) {
 }
 
 Context::~Context() {
+    drop_graph();
+    if (graph_stream) (void)hipStreamDestroy(graph_stream);
     for (hipEvent_t e : probe_events) (void)hipEventDestroy(e);
     for (void* a : allocations) (void)hipFree(a);
 }
@@ -227,9 +229,70 @@ int Context::run(int frames, const void* echo, const double* velocity, const voi
     return run_frames(frames, echo, velocity, nullptr, phi0, img, nullptr, opt, s);
 }
 
+void Context::drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+}
+
 int Context::run_frames(int frames, const void* echo, const double* velocity, const long long* frame_row,
                         const void* phi0, float* img, const long long* frame_img, const isar_options& opt,
                         hipStream_t s) {
+    if (!graph_mode || opt.stage_mode != ISAR_RUN_FULL || probe_stage >= 0)
+        return enqueue(frames, echo, velocity, frame_row, phi0, img, frame_img, opt, s);
+    GraphKey k;
+    k.frames = frames;
+    k.echo = echo;
+    k.velocity = velocity;
+    k.frame_row = frame_row;
+    k.phi0 = phi0;
+    k.img = img;
+    k.frame_img = frame_img;
+    k.opt = opt;
+    k.ra_chunk = ra_chunk;
+    k.af_mode = af_mode;
+    k.hrrp = hrrp_out;
+    const GraphKey& g0 = graph_key;
+    const bool same = graph_exec && k.frames == g0.frames && k.echo == g0.echo && k.velocity == g0.velocity &&
+                      k.frame_row == g0.frame_row && k.phi0 == g0.phi0 && k.img == g0.img &&
+                      k.frame_img == g0.frame_img && k.ra_chunk == g0.ra_chunk && k.af_mode == g0.af_mode &&
+                      k.hrrp == g0.hrrp && std::memcmp(&k.opt, &g0.opt, sizeof(isar_options)) == 0;
+    if (!same) {
+        // capture the launches on an internal stream; the graph is then launched on the caller's
+        drop_graph();
+        if (!graph_stream && hipStreamCreateWithFlags(&graph_stream, hipStreamNonBlocking) != hipSuccess)
+            return fail(ISAR_ERR_DEVICE, "graph stream");
+        if (hipStreamBeginCapture(graph_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return fail(ISAR_ERR_DEVICE, "begin capture");
+        const int rc = enqueue(frames, echo, velocity, frame_row, phi0, img, frame_img, opt, graph_stream);
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(graph_stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess || hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            graph_exec = nullptr;
+            return fail(ISAR_ERR_DEVICE, "graph capture");
+        }
+        graph = g;
+        graph_key = k;
+        graph_job = last_job;
+    }
+    if (hipGraphLaunch(graph_exec, s) != hipSuccess) return fail(ISAR_ERR_DEVICE, "graph launch");
+    // the bookkeeping of the captured run
+    last_frames = frames;
+    front_frames = frames;
+    last_job = graph_job;
+    last_mode = ISAR_RUN_FULL;
+    return ISAR_OK;
+}
+
+int Context::enqueue(int frames, const void* echo, const double* velocity, const long long* frame_row,
+                     const void* phi0, float* img, const long long* frame_img, const isar_options& opt,
+                     hipStream_t s) {
     if (frames < 1 || frames > max_frames)
         return fail(ISAR_ERR_INVALID, "frames must be in [1, " + std::to_string(max_frames) + "]");
     const int mode = opt.stage_mode;
@@ -564,6 +627,13 @@ ISAR_API int isar_set_autofocus_mode(isar_context* ctx, int mode) {
 }
 
 ISAR_API int isar_get_autofocus_mode(isar_context* ctx) { return ctx ? ctx->af_mode : -1; }
+
+ISAR_API int isar_set_graph_mode(isar_context* ctx, int on) {
+    if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");
+    ctx->graph_mode = on != 0;
+    if (!ctx->graph_mode) ctx->drop_graph();
+    return ISAR_OK;
+}
 
 ISAR_API int isar_set_hrrp_output(isar_context* ctx, void* d_hrrp) {
     if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");

@@ -53,6 +53,30 @@ struct Context {
     int run_frames(int frames, const void* echo, const double* velocity, const long long* frame_row,
                    const void* phi0, float* img, const long long* frame_img, const isar_options& opt,
                    hipStream_t stream);
+    // the launches of one run, without graph replay
+    int enqueue(int frames, const void* echo, const double* velocity, const long long* frame_row,
+                const void* phi0, float* img, const long long* frame_img, const isar_options& opt,
+                hipStream_t stream);
+    // graph replay of full runs whose arguments repeat (isar_set_graph_mode)
+    struct GraphKey {
+        int frames = 0;
+        const void* echo = nullptr;
+        const double* velocity = nullptr;
+        const long long* frame_row = nullptr;
+        const void* phi0 = nullptr;
+        float* img = nullptr;
+        const long long* frame_img = nullptr;
+        isar_options opt{};
+        int ra_chunk = 0, af_mode = 0;
+        const float2* hrrp = nullptr;
+    };
+    bool graph_mode = false;
+    GraphKey graph_key{};
+    Job graph_job{};
+    hipStream_t graph_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    void drop_graph();
     // the last run, for isar_stage_read: its job (with the caller's buffer pointers) and stage
     // mode, and scratch for rebuilding intermediate arrays (allocated on the first stage read)
     Job last_job{};
