@@ -1,6 +1,7 @@
 """Summarise a rocprofv3 kernel-trace database (rocpd SQLite) as a markdown table.
 
   python -m isar_imaging_cuda_amd.profile_summary <results.db> "<command that produced it>" [steps]
+  python -m isar_imaging_cuda_amd.profile_summary --counters <results.db>   (rocprofv3 --pmc runs)
 
 Prints per-kernel launches, total and average duration, share of GPU time and, when `steps` is
 given, the time per bench step.
@@ -31,5 +32,30 @@ def summarise(db: str, command: str, steps: int | None = None) -> str:
     return "\n".join(out)
 
 
+def counters(db: str) -> dict[str, dict[str, float]]:
+    """Mean value per dispatch of every collected counter, by kernel, from a rocprofv3 --pmc
+    database: {kernel: {"dispatches": n, "duration_us": mean, counter: mean, ...}}."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select kernel_name, counter_name, dispatch_id, sum(value), max(duration) "
+                       "from counters_collection group by kernel_name, counter_name, dispatch_id").fetchall()
+    acc: dict[str, dict[str, list[float]]] = {}
+    durs: dict[str, dict[int, float]] = {}
+    for name, counter, dispatch, value, duration in rows:
+        short = name.replace("isar::", "").replace("(isar::Job)", "").replace("(isar::Job, int, int)", "")
+        acc.setdefault(short, {}).setdefault(counter, []).append(float(value))
+        durs.setdefault(short, {})[dispatch] = float(duration or 0.0)
+    out = {}
+    for k, cs in acc.items():
+        d = {c: sum(v) / len(v) for c, v in cs.items()}
+        d["dispatches"] = float(max(len(v) for v in cs.values()))
+        d["duration_us"] = sum(durs[k].values()) / max(len(durs[k]), 1) / 1e3
+        out[k] = d
+    return out
+
+
 if __name__ == "__main__":
-    print(summarise(sys.argv[1], sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else None))
+    if sys.argv[1] == "--counters":
+        import json
+        print(json.dumps(counters(sys.argv[2]), indent=1))
+    else:
+        print(summarise(sys.argv[1], sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else None))
