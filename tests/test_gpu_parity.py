@@ -69,6 +69,30 @@ def test_pipeline_parity(name, cid, over):
     assert rep.ok(), str(rep)
 
 
+@pytest.mark.parametrize("cid,seeds", [(1, range(500, 548)), (2, range(600, 616)), (4, range(700, 716))],
+                         ids=["c1x48", "c2x16", "c4x16"])
+def test_pipeline_parity_seed_sweep(cid, seeds):
+    """One batch of frames with different seeds (scene, motion, phase error and noise draws),
+    every frame against the oracle: the batched path over many draws, near-tie decisions
+    included."""
+    cfgs = [synth.preset(cid, seed=s) for s in seeds]
+    frames = [synth.frame(c) for c in cfgs]
+    radar = synth.radar_header(cfgs[0])
+    M, N = cfgs[0].echo_num, cfgs[0].range_num
+    K = len(cfgs)
+    print(f"[case] seed sweep c{cid}: {K} frames {M}x{N} in one batch", flush=True)
+    echo = np.stack([f[0] for f in frames])
+    vel = np.stack([f[1] for f in frames])
+    img, traces = _run_device(echo, vel, radar, M, N, isar.Options(), frames=K)
+    bad = []
+    for k in range(K):
+        img_o, tr = oracle_run(echo[k], vel[k], radar, M, N)
+        rep = compare(img[k], traces[k], img_o, tr, M)
+        if not rep.ok():
+            bad.append((k, str(rep)))
+    assert not bad, bad
+
+
 VARIANTS = [
     ("no_mtrc", dict(if_mtrc=False), dict(if_mtrc=0)),
     ("window_intended", dict(quirk_q1=False), dict(quirk_q1=0)),
