@@ -61,11 +61,16 @@ ISAR_API int isar_compose_phase(int frames, int M, const void* d_phi_b, const fl
 /* Entropy -sum p ln p, p = img^2 / sum img^2, of `frames` images of n floats each (double). */
 ISAR_API int isar_image_entropy(int frames, int n, const float* d_img, double* d_out, void* stream);
 /* The plain autofocus judges the network: per frame, d_out[f] = d_phi_b[f] when its image
- * entropy d_ent_b[f] is lower than d_ent_a[f], else d_phi_a[f] (frames x M complex phases). With
- * phi_a the plain phase after the front iterations and phi_b the warm start, a prediction that
- * does not lower the entropy is dropped and the plain iteration continues. */
+ * entropy d_ent_b[f] is lower than d_ent_a[f], else d_phi_a[f] (frames x M complex phases), and
+ * d_ent_out[f] (optional) the entropy of the phase kept. With phi_a the plain phase after the
+ * front iterations and phi_b a warm start, a prediction that does not lower the entropy is
+ * dropped and the plain iteration continues. Chained, it keeps the lowest of several candidates
+ * (the first on ties). */
 ISAR_API int isar_select_phase(int frames, int M, const void* d_phi_a, const double* d_ent_a, const void* d_phi_b,
-                               const double* d_ent_b, void* d_out, void* stream);
+                               const double* d_ent_b, void* d_out, double* d_ent_out, void* stream);
+/* Half step from phi_a toward phi_b per pulse: phi_a sqrt(phi_b conj(phi_a)) on the shorter arc
+ * (a second warm-start candidate, half of the predicted change). */
+ISAR_API int isar_half_phase(int frames, int M, const void* d_phi_a, const void* d_phi_b, void* d_out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -225,12 +225,19 @@ ISAR_API int isar_compose_phase(int frames, int M, const void* d_phi_b, const fl
 }
 
 ISAR_API int isar_select_phase(int frames, int M, const void* d_phi_a, const double* d_ent_a, const void* d_phi_b,
-                               const double* d_ent_b, void* d_out, void* stream) {
+                               const double* d_ent_b, void* d_out, double* d_ent_out, void* stream) {
     if (frames < 1 || M < 1 || !d_phi_a || !d_ent_a || !d_phi_b || !d_ent_b || !d_out)
         return fail(ISAR_ERR_INVALID, "isar_select_phase: bad arguments");
     isar::launch_select_phase(frames, M, static_cast<const float2*>(d_phi_a), d_ent_a, static_cast<const float2*>(d_phi_b),
-                              d_ent_b, static_cast<float2*>(d_out), static_cast<hipStream_t>(stream));
+                              d_ent_b, static_cast<float2*>(d_out), d_ent_out, static_cast<hipStream_t>(stream));
     return hipGetLastError() == hipSuccess ? ISAR_OK : fail(ISAR_ERR_DEVICE, "isar_select_phase: launch failed");
+}
+
+ISAR_API int isar_half_phase(int frames, int M, const void* d_phi_a, const void* d_phi_b, void* d_out, void* stream) {
+    if (frames < 1 || M < 1 || !d_phi_a || !d_phi_b || !d_out) return fail(ISAR_ERR_INVALID, "isar_half_phase: bad arguments");
+    isar::launch_half_phase(frames, M, static_cast<const float2*>(d_phi_a), static_cast<const float2*>(d_phi_b),
+                            static_cast<float2*>(d_out), static_cast<hipStream_t>(stream));
+    return hipGetLastError() == hipSuccess ? ISAR_OK : fail(ISAR_ERR_DEVICE, "isar_half_phase: launch failed");
 }
 
 ISAR_API int isar_image_entropy(int frames, int n, const float* d_img, double* d_out, void* stream) {

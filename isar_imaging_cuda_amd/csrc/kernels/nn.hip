@@ -175,14 +175,32 @@ __global__ void k_compose_phase(int frames, int M, const float2* __restrict__ ph
 }
 
 // Per frame, the starting phase whose image entropy is lower: out = ent_b < ent_a ? phi_b : phi_a
-// (ties keep phi_a).
+// (ties keep phi_a); ent_out (optional) receives the entropy of the phase kept.
 __global__ void k_select_phase(int frames, int M, const float2* __restrict__ phi_a, const double* __restrict__ ent_a,
                                const float2* __restrict__ phi_b, const double* __restrict__ ent_b,
-                               float2* __restrict__ out) {
+                               float2* __restrict__ out, double* __restrict__ ent_out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= frames * M) return;
     const int f = i / M;
-    out[i] = ent_b[f] < ent_a[f] ? phi_b[i] : phi_a[i];
+    const bool take_b = ent_b[f] < ent_a[f];
+    out[i] = take_b ? phi_b[i] : phi_a[i];
+    if (ent_out && i % M == 0) ent_out[f] = take_b ? ent_b[f] : ent_a[f];
+}
+
+// Half step from phi_a toward phi_b: out = phi_a * sqrt(unit(phi_b conj(phi_a))), the square root
+// taken on the shorter arc (unit(1 + c)); an exactly opposite pair turns by +pi/2.
+__global__ void k_half_phase(int frames, int M, const float2* __restrict__ phi_a, const float2* __restrict__ phi_b,
+                             float2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= frames * M) return;
+    const float2 a = phi_a[i];
+    float2 c = cconjmul(a, phi_b[i]);                  // phi_b conj(phi_a)
+    const float cm = cabsf_(c);
+    c = cm > 0.0f ? cscale(c, 1.0f / cm) : make_float2(1.0f, 0.0f);
+    float2 h = make_float2(1.0f + c.x, c.y);
+    const float hm = cabsf_(h);
+    h = hm > 1e-20f ? cscale(h, 1.0f / hm) : make_float2(0.0f, 1.0f);
+    out[i] = cmul(a, h);
 }
 
 // Image entropy -sum p ln p, p = img^2 / sum img^2, one workgroup per image, FP64 sums.
@@ -262,9 +280,12 @@ void launch_compose_phase(int frames, int M, const float2* phi_b, const float* p
     ISAR_LAUNCH(k_compose_phase, dim3((frames * M + 255) / 256), dim3(256), 0, s, frames, M, phi_b, pred, phi_w);
 }
 void launch_select_phase(int frames, int M, const float2* phi_a, const double* ent_a, const float2* phi_b,
-                         const double* ent_b, float2* out, hipStream_t s) {
+                         const double* ent_b, float2* out, double* ent_out, hipStream_t s) {
     ISAR_LAUNCH(k_select_phase, dim3((frames * M + 255) / 256), dim3(256), 0, s, frames, M, phi_a, ent_a, phi_b, ent_b,
-                out);
+                out, ent_out);
+}
+void launch_half_phase(int frames, int M, const float2* phi_a, const float2* phi_b, float2* out, hipStream_t s) {
+    ISAR_LAUNCH(k_half_phase, dim3((frames * M + 255) / 256), dim3(256), 0, s, frames, M, phi_a, phi_b, out);
 }
 void launch_image_entropy(int frames, int n, const float* img, double* out, hipStream_t s) {
     ISAR_LAUNCH(k_image_entropy, dim3(frames), dim3(256), 0, s, n, img, out);

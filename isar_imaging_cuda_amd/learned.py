@@ -69,7 +69,8 @@ def _bind() -> C.CDLL:
     L.isar_phase_change.argtypes = [i, i, vp, vp, vp, i, vp]
     L.isar_compose_phase.argtypes = [i, i, vp, vp, vp, vp]
     L.isar_image_entropy.argtypes = [i, i, vp, vp, vp]
-    L.isar_select_phase.argtypes = [i, i, vp, vp, vp, vp, vp, vp]
+    L.isar_select_phase.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp]
+    L.isar_half_phase.argtypes = [i, i, vp, vp, vp, vp]
     L._learned_bound = True
     return L
 
@@ -149,9 +150,13 @@ def image_entropy(frames: int, n: int, d_img: int, d_out: int, stream: int | Non
 
 
 def select_phase(frames: int, M: int, d_phi_a: int, d_ent_a: int, d_phi_b: int, d_ent_b: int, d_out: int,
-                 stream: int | None = None) -> None:
+                 stream: int | None = None, d_ent_out: int | None = None) -> None:
     _check(_bind().isar_select_phase(frames, M, _p(d_phi_a), _p(d_ent_a), _p(d_phi_b), _p(d_ent_b), _p(d_out),
-                                     _p(stream)), "isar_select_phase")
+                                     _p(d_ent_out), _p(stream)), "isar_select_phase")
+
+
+def half_phase(frames: int, M: int, d_phi_a: int, d_phi_b: int, d_out: int, stream: int | None = None) -> None:
+    _check(_bind().isar_half_phase(frames, M, _p(d_phi_a), _p(d_phi_b), _p(d_out), _p(stream)), "isar_half_phase")
 
 
 # ------------------------------------------------------------------------------------------
@@ -159,8 +164,8 @@ def select_phase(frames: int, M: int, d_phi_a: int, d_ent_a: int, d_phi_b: int, 
 # ------------------------------------------------------------------------------------------
 class WarmStartImager:
     """An imaging context plus the network: front (FRONT_ITERS plain iterations) -> network ->
-    judge (keep the prediction only where its image entropy is lower than the plain phase's) ->
-    back (the iteration from the chosen start, then the image)."""
+    judge (per frame, the start of lowest image entropy among the plain phase, the prediction and
+    the half step toward it) -> back (the iteration from the chosen start, then the image)."""
 
     def __init__(self, params: RadarParams, max_frames: int, net_params: np.ndarray | None = None,
                  front_iters: int = FRONT_ITERS):
@@ -182,8 +187,13 @@ class WarmStartImager:
         self.feat = torch.zeros((max_frames, 2 * self.M * front_iters), dtype=torch.float32, device=dev)
         self.pred = torch.zeros((max_frames, 2 * self.M), dtype=torch.float32, device=dev)
         self.phi_s = torch.zeros_like(self.phis[0])      # the judged start
+        self.phi_h = torch.zeros_like(self.phis[0])      # half step toward the prediction
+        self.phi_t = torch.zeros_like(self.phis[0])      # the better of the plain phase and the prediction
         self.ent_plain = torch.zeros(max_frames, dtype=torch.float64, device=dev)
         self.ent_warm = torch.zeros(max_frames, dtype=torch.float64, device=dev)
+        self.ent_half = torch.zeros(max_frames, dtype=torch.float64, device=dev)
+        self.ent_t = torch.zeros(max_frames, dtype=torch.float64, device=dev)
+        self.ent_start = torch.zeros(max_frames, dtype=torch.float64, device=dev)
 
     @property
     def phi_last(self):
@@ -218,15 +228,18 @@ class WarmStartImager:
         self.imager.run_device(0, 0, img_ptr, K, o, phi0_ptr=start, stream=stream)
 
     def judge(self, K: int, img_ptr: int, stream: int) -> None:
-        """The plain objective decides: image entropy from phi_F and from phi_w (img_ptr: K x M x W
-        floats of scratch), phi_s = the start with the lower entropy per frame."""
+        """The plain objective decides: image entropies from phi_F, from the prediction phi_w and
+        from the half step toward it (img_ptr: K x M x W floats of scratch); phi_s = the start of
+        lowest entropy per frame (the first of them on ties), ent_start its entropy."""
         n = self.M * self.W
-        self.back(K, img_ptr, 0, stream, phi=self.phi_last)
-        image_entropy(K, n, img_ptr, self.ent_plain.data_ptr(), stream)
-        self.back(K, img_ptr, 0, stream, phi=self.phi_w)
-        image_entropy(K, n, img_ptr, self.ent_warm.data_ptr(), stream)
+        half_phase(K, self.M, self.phi_last.data_ptr(), self.phi_w.data_ptr(), self.phi_h.data_ptr(), stream)
+        for phi, ent in ((self.phi_last, self.ent_plain), (self.phi_w, self.ent_warm), (self.phi_h, self.ent_half)):
+            self.back(K, img_ptr, 0, stream, phi=phi)
+            image_entropy(K, n, img_ptr, ent.data_ptr(), stream)
         select_phase(K, self.M, self.phi_last.data_ptr(), self.ent_plain.data_ptr(), self.phi_w.data_ptr(),
-                     self.ent_warm.data_ptr(), self.phi_s.data_ptr(), stream)
+                     self.ent_warm.data_ptr(), self.phi_t.data_ptr(), stream, self.ent_t.data_ptr())
+        select_phase(K, self.M, self.phi_t.data_ptr(), self.ent_t.data_ptr(), self.phi_h.data_ptr(),
+                     self.ent_half.data_ptr(), self.phi_s.data_ptr(), stream, self.ent_start.data_ptr())
 
     def run(self, K: int, echo_ptr: int, vel_ptr: int, img_ptr: int, iterations: int, stream: int) -> None:
         """The learned chain: F + `iterations` autofocus iterations in total."""
@@ -351,7 +364,7 @@ def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int
         ws.front(K, e.data_ptr(), v.data_ptr(), s)
         ws.predict(K, s)
         ws.judge(K, img.data_ptr(), s)
-        kept.append((ws.ent_warm[:K] < ws.ent_plain[:K]).cpu().numpy())
+        kept.append((ws.ent_start[:K] < ws.ent_plain[:K]).cpu().numpy())
         curve_p = np.zeros((K, max_iter + 1))
         curve_l = np.zeros((K, max_iter + 1))
         for k in range(max_iter + 1):

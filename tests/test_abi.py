@@ -74,8 +74,10 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
     assert L.isar_stage_bytes(None, 0) == 0
     L.isar_stage_read.argtypes = [vp, C.c_int, C.c_int, vp]
     assert L.isar_stage_read(None, 0, 0, dummy) == -1
-    L.isar_select_phase.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp]
-    assert L.isar_select_phase(0, 256, dummy, dummy, dummy, dummy, dummy, None) == -1
+    L.isar_select_phase.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp, vp, vp, vp]
+    assert L.isar_select_phase(0, 256, dummy, dummy, dummy, dummy, dummy, None, None) == -1
+    L.isar_half_phase.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]
+    assert L.isar_half_phase(4, 256, dummy, None, dummy, None) == -1
     # the drop-in session's extras before imagingMemInit: out of order
     L.isar_session_hrrp.argtypes = [vp]
     assert L.isar_session_hrrp(dummy) == -4

@@ -162,11 +162,21 @@ def test_phase_helpers_match_numpy():
     dea = torch.from_numpy(ea).to(dev)
     deb = torch.from_numpy(eb).to(dev)
     sel = torch.zeros((K, 2 * M), dtype=torch.float32, device=dev)
-    L.select_phase(K, M, da.data_ptr(), dea.data_ptr(), db.data_ptr(), deb.data_ptr(), sel.data_ptr(), s)
+    ent_kept = torch.zeros(K, dtype=torch.float64, device=dev)
+    L.select_phase(K, M, da.data_ptr(), dea.data_ptr(), db.data_ptr(), deb.data_ptr(), sel.data_ptr(), s,
+                   ent_kept.data_ptr())
     torch.cuda.synchronize()
     got = sel.cpu().numpy().view(np.complex64)
     for k, take_b in enumerate([True, False, False, True, True]):
         assert np.array_equal(got[k], b[k] if take_b else a[k]), f"frame {k}"
+    assert np.array_equal(ent_kept.cpu().numpy(), np.where(eb < ea, eb, ea))
+    # the half step: angle of b conj(a) halved, the shorter way round
+    half = torch.zeros((K, 2 * M), dtype=torch.float32, device=dev)
+    L.half_phase(K, M, da.data_ptr(), db.data_ptr(), half.data_ptr(), s)
+    torch.cuda.synchronize()
+    d = np.angle(b * np.conj(a))
+    ref = a * np.exp(0.5j * d)
+    assert np.abs(half.cpu().numpy().view(np.complex64) - ref).max() < 1e-5   # two FP32 normalisations
 
 
 @pytest.mark.gpu
