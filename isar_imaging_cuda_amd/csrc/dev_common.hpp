@@ -25,6 +25,14 @@ __device__ __forceinline__ float2 cconjmul(float2 a, float2 b) {
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ __forceinline__ float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
 __device__ __forceinline__ float cabsf_(float2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
+// float -> int as a CUDA device conversion performs it: truncation toward zero, clamped to the int
+// range, NaN -> 0 (the reference relies on it where Thrust accumulates floats as ints)
+__device__ __forceinline__ int trunc_sat_i32(float v) {
+    if (!(v == v)) return 0;
+    if (v >= 2147483648.0f) return 2147483647;
+    if (v <= -2147483648.0f) return -2147483647 - 1;
+    return static_cast<int>(v);
+}
 // multiply by (dir * i): dir = -1 -> -i, dir = +1 -> +i
 template <int DIR>
 __device__ __forceinline__ float2 cmul_i(float2 a) {
@@ -174,23 +182,24 @@ __device__ __forceinline__ void block_argmin(float& v, int& i, float* redv, int*
         if (ov < v || (ov == v && oi < i)) { v = ov; i = oi; }
     }
 }
-// Exclusive prefix sum of ints over the workgroup; returns the exclusive prefix, total in *tot.
-template <int NT>
-__device__ __forceinline__ int block_exclusive_scan(int v, int* red, int* tot) {
+// Exclusive prefix sum of integers over the workgroup; returns the exclusive prefix, total in
+// *tot. With T = unsigned the sums wrap modulo 2^32.
+template <int NT, typename T = int>
+__device__ __forceinline__ T block_exclusive_scan(T v, T* red, T* tot) {
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    int x = v;
+    T x = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        int y = __shfl_up(x, o, 64);
+        T y = __shfl_up(x, o, 64);
         if (l >= o) x += y;
     }
     __syncthreads();
     if (l == 63) red[w] = x;
     __syncthreads();
-    int base = 0, all = 0;
+    T base = 0, all = 0;
 #pragma unroll
     for (int k = 0; k < NT / 64; ++k) {
-        int r = red[k];
+        T r = red[k];
         if (k < w) base += r;
         all += r;
     }

@@ -119,9 +119,12 @@ __global__ __launch_bounds__(W) void k_pa_select(Job J) {
     const int f = blockIdx.x, n = threadIdx.x;
     const float mx = J.colmax[static_cast<size_t>(f) * W + n];
     const float E = J.colE[static_cast<size_t>(f) * W + n];
-    // thrust::reduce(..., 0, plus<int>()) truncates every maximum to int
-    int tot = 0;
-    (void)block_exclusive_scan<W>(static_cast<int>(mx), ired, &tot);
+    // thrust::reduce(..., 0, plus<int>()) (phase_adjustment.cu:186) converts every maximum to int
+    // as the device does (truncation, clamped) and adds them in 32-bit arithmetic, which wraps
+    unsigned utot = 0;
+    (void)block_exclusive_scan<W, unsigned>(static_cast<unsigned>(trunc_sat_i32(mx)),
+                                            reinterpret_cast<unsigned*>(ired), &utot);
+    const int tot = static_cast<int>(utot);   // two's complement
     const float mean_val = static_cast<float>(tot) / static_cast<float>(W);
     const float threshold = 1.48f * mean_val;
     int tgt = 0;
@@ -195,10 +198,18 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     float2* su = J.su + static_cast<size_t>(f) * W * M;
     const float2* X = J.x + static_cast<size_t>(f) * M * W;
 
-    // gather the selected bins, column-major (selectRangeBins, :335-346)
+    // gather the selected bins, column-major (selectRangeBins, :335-346), scaled by 2^-e so that
+    // the largest selected |S| lies in [1, 2): the update takes the unit phasor of sums cubic in
+    // S, and a power of two scales every product exactly, so the phases are those of the unscaled
+    // bins while conj(S) IFFT(G |G|) stays far from FP32 overflow at any input level
+    float cm = 0.0f;
+    for (int u = tid; u < U; u += NT) cm = fmaxf(cm, J.colmax[static_cast<size_t>(f) * W + sel[u]]);
+    cm = block_max<NT>(cm, reinterpret_cast<float*>(red));
+    const int e = cm > 0.0f && cm <= 3.40282347e38f ? ilogbf(cm) : 0;
     for (int idx = tid; idx < U * M; idx += NT) {
         const int m = idx / U, u = idx % U;
-        su[static_cast<size_t>(u) * M + m] = X[static_cast<size_t>(m) * W + sel[u]];
+        const float2 v = X[static_cast<size_t>(m) * W + sel[u]];
+        su[static_cast<size_t>(u) * M + m] = make_float2(ldexpf(v.x, -e), ldexpf(v.y, -e));
     }
     __syncthreads();
     if (J.phi0_given) {

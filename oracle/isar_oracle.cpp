@@ -484,9 +484,16 @@ void orc_fast_entropy(double* x_, int M, int W, int iterations, int phi0_given, 
     std::vector<double> mx(W, 0.0);
     for (int m = 0; m < M; ++m)
         for (int n = 0; n < W; ++n) mx[n] = std::max(mx[n], std::abs(X[static_cast<size_t>(m) * W + n]));
-    long long isum = 0;
-    for (int n = 0; n < W; ++n) isum += static_cast<long long>(static_cast<int>(mx[n]));
-    const float mean_val = static_cast<float>(static_cast<int>(isum)) / static_cast<float>(W);
+    // :186 thrust::reduce(..., 0, plus<int>()): each maximum becomes an int as CUDA device code
+    // converts it (truncation toward zero, clamped to the int range), and the ints add in 32-bit
+    // arithmetic, which wraps modulo 2^32
+    uint32_t usum = 0;
+    for (int n = 0; n < W; ++n) {
+        const double v = mx[n];
+        const int32_t t = !(v == v) ? 0 : v >= 2147483648.0 ? INT32_MAX : static_cast<int32_t>(v);
+        usum += static_cast<uint32_t>(t);
+    }
+    const float mean_val = static_cast<float>(static_cast<int32_t>(usum)) / static_cast<float>(W);
     const float threshold = 1.48f * mean_val;
     // :190-194 tgt_num
     int tgt_num = 0;

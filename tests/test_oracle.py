@@ -292,6 +292,24 @@ def test_bin_selection_matches_hand_ranking():
     assert np.allclose(tr.pa_colE[:W], E) and np.allclose(tr.pa_colH[:W], H)
 
 
+@pytest.mark.parametrize("level, want_sum", [
+    (1e7, 512 * 10_000_000 - 2**32),   # 5.12e9 wraps once modulo 2^32
+    (3e9, -512),                       # each maximum clamps to 2^31 - 1; 512 (2^31 - 1) = -512 mod 2^32
+])
+def test_threshold_mean_wraps_like_int32_thrust(level, want_sum):
+    # phase_adjustment.cu:186 accumulates the column maxima through thrust::plus<int>: each float
+    # is truncated and clamped to int on conversion and the int sum wraps; with every column
+    # maximum equal to `level` the mean is the wrapped sum / W
+    M, W = 16, 512
+    X = np.full((M, W), 0.25 * level, complex)
+    X[3, :] = level
+    tr = O.Trace(M, 2, W)
+    O.fast_entropy(X, iterations=0, trace=tr)
+    mean = np.float32(np.float32(want_sum) / np.float32(W))
+    assert tr.c.pa_threshold == pytest.approx(float(np.float32(1.48) * mean), rel=1e-7)
+    assert tr.c.tgt_num == (W if want_sum < 0 or level > 1.48 * want_sum / W else 0)
+
+
 # ---------------------------------------------------------------------------------------------
 # whole chain
 # ---------------------------------------------------------------------------------------------
