@@ -222,6 +222,8 @@ ISAR_API int isar_probe_enable(isar_context* ctx, int stage);
  * the last enable/read, then resets the accumulator. */
 ISAR_API int isar_probe_read(isar_context* ctx, double* total_ms, int* launches);
 
+/* Message of the last failure on the calling thread (each thread keeps its own, like errno);
+ * valid until that thread's next failing call. */
 ISAR_API const char* isar_last_error(void);
 ISAR_API int isar_device_count(void);
 

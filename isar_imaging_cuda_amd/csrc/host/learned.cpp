@@ -1,5 +1,6 @@
 // learned.cpp — the warm-start network object behind isar_learned.h: FP32 master parameters,
 // BF16 weight copies, Adam moments and per-layer activations, all allocated at creation.
+#include <climits>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -39,13 +40,21 @@ struct isar_mlp : isar::Mlp {};
 
 using isar::fail;
 
+namespace {
+// The element-wise phase kernels index frames x M, and the GEMMs batch x width, with 32-bit ints
+// (grid size included): larger requests are refused rather than wrapped.
+bool int_indexable(long long a, long long b) { return a * b <= static_cast<long long>(INT_MAX) - 1024; }
+}  // namespace
+
 extern "C" {
 
 ISAR_API int isar_mlp_create(int n_layers, const int* widths, int max_batch, isar_mlp** out) {
     if (!out || !widths || n_layers < 1 || n_layers > 16 || max_batch < 1)
         return fail(ISAR_ERR_INVALID, "isar_mlp_create: bad arguments");
-    for (int l = 0; l <= n_layers; ++l)
+    for (int l = 0; l <= n_layers; ++l) {
         if (widths[l] < 1 || widths[l] > 8192) return fail(ISAR_ERR_INVALID, "isar_mlp_create: width out of range");
+        if (!int_indexable(max_batch, widths[l])) return fail(ISAR_ERR_INVALID, "isar_mlp_create: max_batch x width too large");
+    }
     auto* net = new isar_mlp();
     net->widths.assign(widths, widths + n_layers + 1);
     net->max_batch = max_batch;
@@ -208,7 +217,7 @@ ISAR_API int isar_mse(int rows, int cols, const float* d_pred, const float* d_ta
 
 ISAR_API int isar_phase_change(int frames, int M, const void* d_phi_a, const void* d_phi_b, float* d_out,
                                int out_stride, void* stream) {
-    if (frames < 1 || M < 1 || !d_phi_a || !d_phi_b || !d_out || out_stride < 2 * M)
+    if (frames < 1 || M < 1 || !int_indexable(frames, M) || !d_phi_a || !d_phi_b || !d_out || out_stride < 2LL * M)
         return fail(ISAR_ERR_INVALID, "isar_phase_change: bad arguments");
     isar::launch_phase_change(frames, M, static_cast<const float2*>(d_phi_a), static_cast<const float2*>(d_phi_b),
                               d_out, out_stride, static_cast<hipStream_t>(stream));
@@ -217,7 +226,7 @@ ISAR_API int isar_phase_change(int frames, int M, const void* d_phi_a, const voi
 
 ISAR_API int isar_compose_phase(int frames, int M, const void* d_phi_b, const float* d_pred, void* d_phi_w,
                                 void* stream) {
-    if (frames < 1 || M < 1 || !d_phi_b || !d_pred || !d_phi_w)
+    if (frames < 1 || M < 1 || !int_indexable(frames, M) || !d_phi_b || !d_pred || !d_phi_w)
         return fail(ISAR_ERR_INVALID, "isar_compose_phase: bad arguments");
     isar::launch_compose_phase(frames, M, static_cast<const float2*>(d_phi_b), d_pred, static_cast<float2*>(d_phi_w),
                                static_cast<hipStream_t>(stream));
@@ -226,7 +235,7 @@ ISAR_API int isar_compose_phase(int frames, int M, const void* d_phi_b, const fl
 
 ISAR_API int isar_select_phase(int frames, int M, const void* d_phi_a, const double* d_ent_a, const void* d_phi_b,
                                const double* d_ent_b, void* d_out, double* d_ent_out, void* stream) {
-    if (frames < 1 || M < 1 || !d_phi_a || !d_ent_a || !d_phi_b || !d_ent_b || !d_out)
+    if (frames < 1 || M < 1 || !int_indexable(frames, M) || !d_phi_a || !d_ent_a || !d_phi_b || !d_ent_b || !d_out)
         return fail(ISAR_ERR_INVALID, "isar_select_phase: bad arguments");
     isar::launch_select_phase(frames, M, static_cast<const float2*>(d_phi_a), d_ent_a, static_cast<const float2*>(d_phi_b),
                               d_ent_b, static_cast<float2*>(d_out), d_ent_out, static_cast<hipStream_t>(stream));
@@ -234,7 +243,7 @@ ISAR_API int isar_select_phase(int frames, int M, const void* d_phi_a, const dou
 }
 
 ISAR_API int isar_half_phase(int frames, int M, const void* d_phi_a, const void* d_phi_b, void* d_out, void* stream) {
-    if (frames < 1 || M < 1 || !d_phi_a || !d_phi_b || !d_out) return fail(ISAR_ERR_INVALID, "isar_half_phase: bad arguments");
+    if (frames < 1 || M < 1 || !int_indexable(frames, M) || !d_phi_a || !d_phi_b || !d_out) return fail(ISAR_ERR_INVALID, "isar_half_phase: bad arguments");
     isar::launch_half_phase(frames, M, static_cast<const float2*>(d_phi_a), static_cast<const float2*>(d_phi_b),
                             static_cast<float2*>(d_out), static_cast<hipStream_t>(stream));
     return hipGetLastError() == hipSuccess ? ISAR_OK : fail(ISAR_ERR_DEVICE, "isar_half_phase: launch failed");

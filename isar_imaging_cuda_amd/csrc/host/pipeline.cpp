@@ -4,7 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <mutex>
+#include <mutex>   // std::call_once
 #include <string>
 #include <vector>
 
@@ -18,26 +18,17 @@ hipError_t configure_ifds_kernels();
 hipError_t configure_debug_kernels();
 
 namespace {
-std::mutex g_err_mu;
-std::string g_last_error;
+thread_local std::string t_last_error;   // per calling thread, like errno
 constexpr double kLightSpeed = 3e8;   // LIGHT_SPEED (common.cuh:69)
 constexpr double kPi = 3.14159265358979323846;
 }  // namespace
 
-void set_error(const std::string& msg) {
-    std::lock_guard<std::mutex> lk(g_err_mu);
-    g_last_error = msg;
-}
+void set_error(const std::string& msg) { t_last_error = msg; }
 int fail(int code, const std::string& msg) {
     set_error(msg);
     return code;
 }
-const char* last_error_cstr() {
-    thread_local std::string copy;
-    std::lock_guard<std::mutex> lk(g_err_mu);
-    copy = g_last_error;
-    return copy.c_str();
-}
+const char* last_error_cstr() { return t_last_error.c_str(); }
 
 std::string g_config_error;
 void report_config_error(const char* file, size_t lds_bytes) {

@@ -78,11 +78,45 @@ def test_new_entry_points_reject_bad_arguments_without_a_device():
     assert L.isar_select_phase(0, 256, dummy, dummy, dummy, dummy, dummy, None, None) == -1
     L.isar_half_phase.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]
     assert L.isar_half_phase(4, 256, dummy, None, dummy, None) == -1
+    # frames x M beyond 32-bit indexing is refused, not wrapped
+    assert L.isar_half_phase(1 << 22, 1024, dummy, dummy, dummy, None) == -1
+    assert L.isar_select_phase(1 << 22, 1024, dummy, dummy, dummy, dummy, dummy, None, None) == -1
+    L.isar_phase_change.argtypes = [C.c_int, C.c_int, vp, vp, vp, C.c_int, vp]
+    assert L.isar_phase_change(1 << 22, 1024, dummy, dummy, dummy, 2048, None) == -1
+    assert L.isar_phase_change(1, 1 << 30, dummy, dummy, dummy, 2048, None) == -1
+    L.isar_compose_phase.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]
+    assert L.isar_compose_phase(1 << 22, 1024, dummy, dummy, dummy, None) == -1
+    L.isar_mlp_create.argtypes = [C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(vp)]
+    widths = (C.c_int * 3)(2048, 512, 2048)
+    net = vp()
+    assert L.isar_mlp_create(2, widths, 1 << 20, C.byref(net)) == -1 and not net.value
+    assert "too large" in L.isar_last_error().decode()
     # the drop-in session's extras before imagingMemInit: out of order
     L.isar_session_hrrp.argtypes = [vp]
     assert L.isar_session_hrrp(dummy) == -4
     L.isar_session_dump.argtypes = [C.c_uint, C.c_char_p]
     assert L.isar_session_dump(0x7F, b"/tmp") == -4
+
+
+def test_last_error_is_per_thread():
+    import threading
+    L = isar.lib()
+    vp = C.c_void_p
+    L.isar_half_phase.argtypes = [C.c_int, C.c_int, vp, vp, vp, vp]
+    assert L.isar_half_phase(0, 256, None, None, None, None) == -1
+    mine = L.isar_last_error().decode()
+    seen = []
+
+    def other():
+        L.isar_stage_read.argtypes = [vp, C.c_int, C.c_int, vp]
+        assert L.isar_stage_read(None, 0, 0, vp(16)) == -1
+        seen.append(L.isar_last_error().decode())
+
+    t = threading.Thread(target=other)
+    t.start()
+    t.join()
+    assert seen and seen[0] != mine and "context" in seen[0]
+    assert L.isar_last_error().decode() == mine
 
 
 def test_workspace_bytes_scales_with_frames():
