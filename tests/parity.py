@@ -41,8 +41,10 @@ def oracle_run(echo, velocity, radar: dict, M: int, N: int, **opts):
 
 
 def rel_l2(a, ref) -> float:
-    """||a - ref|| / ||ref|| in float64; 0 when both are zero, inf when only ref is."""
-    a, ref = np.asarray(a, np.float64), np.asarray(ref, np.float64)
+    """||a - ref|| / ||ref|| in float64 (complex128 for complex data); 0 when both are zero, inf
+    when only ref is."""
+    dt = np.complex128 if np.iscomplexobj(a) or np.iscomplexobj(ref) else np.float64
+    a, ref = np.asarray(a, dt), np.asarray(ref, dt)
     num, den = float(np.linalg.norm(a - ref)), float(np.linalg.norm(ref))
     if den > 0:
         return num / den

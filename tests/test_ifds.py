@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from parity import rel_l2
 
 ROOT = Path(__file__).resolve().parents[1]
 APP = ROOT / "isar_imaging_cuda_amd" / "lib" / "isar_calling_app"
@@ -112,6 +113,9 @@ def test_device_pulse_compression_matches_oracle():
     h = synth.radar_header(cfg)
     dev = torch.device("cuda:0")
     raw = np.stack([iq.real, iq.imag], axis=-1).astype(np.int16)     # int16-valued samples
+    raw[0] = 0                                                        # an all-zero echo
+    raw[1] = np.clip(raw[1].astype(np.int64) * 1000, -32768, 32767)   # clipped at int16 full scale
+    iq = raw[..., 0].astype(np.float64) + 1j * raw[..., 1]
     d_iq = torch.from_numpy(raw).to(dev)
     d_v = torch.from_numpy(vel).to(dev)
     out = torch.zeros((M, 4096, 2), dtype=torch.float32, device=dev)
@@ -131,8 +135,9 @@ def test_device_pulse_compression_matches_oracle():
     for m in range(M):
         ref, off = O.pulse_compress(iq[m], vel[m], op)
         assert go[m] == off, f"echo {m}: cut offset {go[m]} vs {off}"
-        rel = np.linalg.norm(got[m] - ref) / np.linalg.norm(ref)
+        rel = rel_l2(got[m], ref)
         assert rel < 2e-5, f"echo {m}: rel error {rel:.2e}"
+    assert not got[0].any() and go[0] == 0
 
 
 @pytest.mark.gpu
