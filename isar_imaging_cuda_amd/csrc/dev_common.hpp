@@ -25,6 +25,17 @@ __device__ __forceinline__ float2 cconjmul(float2 a, float2 b) {
 __device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
 __device__ __forceinline__ float cabs2(float2 a) { return a.x * a.x + a.y * a.y; }
 __device__ __forceinline__ float cabsf_(float2 a) { return sqrtf(a.x * a.x + a.y * a.y); }
+// Square root by the hardware instruction alone (within 1 ulp). sqrtf compiles to that
+// instruction plus about 15 more for correct rounding; where a square root is taken per element
+// inside FP32 transforms, that expansion costs more issue slots than it buys accuracy.
+__device__ __forceinline__ float sqrt_hw(float v) {
+#ifdef ISAR_EMU
+    return sqrtf(v);
+#else
+    return __builtin_amdgcn_sqrtf(v);
+#endif
+}
+__device__ __forceinline__ float cabs_hw(float2 a) { return sqrt_hw(a.x * a.x + a.y * a.y); }
 // float -> int as a CUDA device conversion performs it: truncation toward zero, clamped to the int
 // range, NaN -> 0 (the reference relies on it where Thrust accumulates floats as ints)
 __device__ __forceinline__ int trunc_sat_i32(float v) {

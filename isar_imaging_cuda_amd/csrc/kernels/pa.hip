@@ -251,7 +251,7 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
             for (int e = 0; e < E; ++e) x[e] = cmul(phi[C::in(lane, e)], s[e]);
             fft<M, E, -1>(x, buf, lane, tw);
 #pragma unroll
-            for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabsf_(x[e]));
+            for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabs_hw(x[e]));
             relayout_out_to_in<M, E>(x, buf, lane);
             fft<M, E, +1>(x, buf, lane, tw);
 #pragma unroll
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(AfCfg<M>::SUB * AfCfg<M>::TPF) void k_af_cols(Job J
     for (int e = 0; e < E; ++e) x[e] = cmul(phi[C::in(lane, e)], s[e]);
     fft<M, E, -1>(x, buf, lane, tw);
 #pragma unroll
-    for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabsf_(x[e]));
+    for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabs_hw(x[e]));
     relayout_out_to_in<M, E>(x, buf, lane);
     fft<M, E, +1>(x, buf, lane, tw);
 #pragma unroll

@@ -114,7 +114,7 @@ __device__ float correlate_pair(const float (&a)[kE], const float (&b)[kE], floa
     constexpr int NT = N / kE;
     float2 x[kE];
 #pragma unroll
-    for (int e = 0; e < kE; ++e) x[e] = make_float2(sqrtf(a[e]), sqrtf(b[e]));
+    for (int e = 0; e < kE; ++e) x[e] = make_float2(sqrt_hw(a[e]), sqrt_hw(b[e]));
     fft<N, kE, -1>(x, lds, tid, tw);
 #pragma unroll
     for (int e = 0; e < kE; ++e) lds[lds_pad(out_pos<N>(tid, e))] = x[e];
