@@ -249,7 +249,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
             const float2 v = cscale(x[e], invN);
             const int pos = out_pos<N>(t, e);
             s0a[pos] = v;
-            const float av = cabsf_(v);
+            const float av = cabs_hw(v);
             ap[pos] = av;
             mx = fmaxf(mx, av);
         }
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
             const float2 v = cscale(x[e], invN);
             const int pos = out_pos<N>(t, e);
             s0b[pos] = v;
-            const float av = cabsf_(v);
+            const float av = cabs_hw(v);
             ap[pos] += av;
             mx = fmaxf(mx, av);
         }
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_upd(Job J, int s, int pair_bas
                 const float2 v = cscale(x[e], invN);
                 const int pos = out_pos<N>(t, e);
                 st[pos] = v;
-                const float av = cabsf_(v);
+                const float av = cabs_hw(v);
                 ap[pos] = (m == r0 ? 0.0f : ap[pos]) + av;
                 mx = fmaxf(mx, av);
             }
