@@ -46,6 +46,8 @@ CASES = [
     ("c1_M512", 1, dict(echo_num=512)),
     ("c2", 2, {}),
     ("c4", 4, {}),                          # 256 x 2048, range-aligned scene
+    ("c1_N8192", 1, dict(range_num=8192)),   # the 8192-cell range transforms
+    ("c1_M256_N8192", 1, dict(echo_num=256, range_num=8192)),
     ("c5", 5, {}),                          # 1024 x 16384, 1000 scatterers, 7 km/s
 ]
 
