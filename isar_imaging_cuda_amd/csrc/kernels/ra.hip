@@ -329,14 +329,20 @@ __global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base)
 // and accumulate their new energies; each workgroup handles up to J.ra_chunk rows. The rows of
 // pair 0 move for the last time at this level: their aligned profiles replace their spectra and
 // their column sums feed HRRP centring.
-template <int N>
-__global__ __launch_bounds__(N / kE, 4) void k_ra_upd(Job J, int s, int pair_base) {
+// PART selects the grid: kUpdAll, the chunks of every pair (final rows where p = 0, one launch
+// per level); kUpdFinal, the chunks of pair 0 only; kUpdRest, those of pairs 1, 2, ... Apart
+// from the final-row work the three run the same code; split into kUpdFinal + kUpdRest, the
+// other pairs, most of the rows, do not carry that work's registers, at the cost of a second
+// launch per level.
+constexpr int kUpdAll = 0, kUpdFinal = 1, kUpdRest = 2;
+template <int N, int PART>
+__global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) void k_ra_upd(Job J, int s, int pair_base) {
     HIP_DYNAMIC_SHARED(float2, lds);
     const int b = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
     const int M = J.M;
     const int R = min(s, J.ra_chunk);
     const int cpp = (s + R - 1) / R;
-    const int p = b / cpp, i = b % cpp;
+    const int p = b / cpp + (PART == kUpdRest ? 1 : 0), i = b % cpp;
     const int gend = min((2 * p + 2) * s, M);
     const int r0 = (2 * p + 1) * s + i * R;
     const int r1 = min(r0 + R, gend);
@@ -344,7 +350,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_upd(Job J, int s, int pair_bas
     const float lag = J.lags[static_cast<size_t>(f) * (M - 1) + pair_base + p];
     const float inv = 1.0f / (static_cast<float>(N) * static_cast<float>(N));
     const float invN = 1.0f / N;
-    const bool final_rows = (p == 0);
+    const bool final_rows = PART == kUpdFinal || (PART == kUpdAll && p == 0);
     float acc[kE];
     float mx = 0.0f;
 #pragma unroll
@@ -423,7 +429,9 @@ hipError_t configure_ra_kernels() {
     set(reinterpret_cast<const void*>(&k_hrrp<NN>), ra_lds<NN>());             \
     set(reinterpret_cast<const void*>(&k_ra_pair0<NN>), ra_lds<NN>());         \
     set(reinterpret_cast<const void*>(&k_ra_corr<NN>), ra_lds<NN>());          \
-    set(reinterpret_cast<const void*>(&k_ra_upd<NN>), ra_lds<NN>());
+    set(reinterpret_cast<const void*>(&k_ra_upd<NN, kUpdAll>), ra_lds<NN>());   \
+    set(reinterpret_cast<const void*>(&k_ra_upd<NN, kUpdFinal>), ra_lds<NN>()); \
+    set(reinterpret_cast<const void*>(&k_ra_upd<NN, kUpdRest>), ra_lds<NN>());
     ISAR_SET_RA(1024) ISAR_SET_RA(2048) ISAR_SET_RA(4096) ISAR_SET_RA(8192) ISAR_SET_RA(16384)
 #undef ISAR_SET_RA
 #endif
@@ -445,8 +453,20 @@ void launch_ra_corr(const Job& J, int stride, int pair_base, int nblocks, hipStr
                                        stride, pair_base));
 }
 void launch_ra_upd(const Job& J, int stride, int pair_base, int nblocks, hipStream_t s) {
-    ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH(k_ra_upd<NN>, dim3(nblocks, J.K), dim3(NN / kE), ra_lds<NN>(), s, J,
+    // nblocks = pairs x chunks per pair. Large batches launch pair 0 (final rows) and the other
+    // pairs apart; small ones, where the second launch per level costs more than it saves, in one.
+    if (J.K <= kRaSplitFrames) {
+        ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH((k_ra_upd<NN, kUpdAll>), dim3(nblocks, J.K), dim3(NN / kE), ra_lds<NN>(),
+                                           s, J, stride, pair_base));
+        return;
+    }
+    const int R = std::min(stride, J.ra_chunk), cpp = (stride + R - 1) / R;
+    const int rest = nblocks - cpp;
+    ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH((k_ra_upd<NN, kUpdFinal>), dim3(cpp, J.K), dim3(NN / kE), ra_lds<NN>(), s, J,
                                        stride, pair_base));
+    if (rest > 0)
+        ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH((k_ra_upd<NN, kUpdRest>), dim3(rest, J.K), dim3(NN / kE), ra_lds<NN>(), s,
+                                           J, stride, pair_base));
 }
 
 }  // namespace isar
