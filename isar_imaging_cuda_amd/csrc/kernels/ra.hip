@@ -455,7 +455,7 @@ void launch_ra_corr(const Job& J, int stride, int pair_base, int nblocks, hipStr
 void launch_ra_upd(const Job& J, int stride, int pair_base, int nblocks, hipStream_t s) {
     // nblocks = pairs x chunks per pair. Large batches launch pair 0 (final rows) and the other
     // pairs apart; small ones, where the second launch per level costs more than it saves, in one.
-    if (J.K <= kRaSplitFrames) {
+    if (static_cast<long long>(J.K) * J.M * J.N < kRaSplitCells) {
         ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH((k_ra_upd<NN, kUpdAll>), dim3(nblocks, J.K), dim3(NN / kE), ra_lds<NN>(),
                                            s, J, stride, pair_base));
         return;

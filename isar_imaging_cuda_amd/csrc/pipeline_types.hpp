@@ -18,7 +18,9 @@ constexpr int kRangeNumIfdsPc = 4096; // RANGE_NUM_IFDS_PC (common.cuh:72)
 constexpr int kRaChunkRowsDefault = 8;   // rows per workgroup in the alignment update kernel
 constexpr int kCutBandRows = 16;      // rows per workgroup when cutting the range window
 constexpr int kAfSplitFrames = 32;    // batches of at most this many frames split the autofocus
-constexpr int kRaSplitFrames = 32;    // larger batches update the final rows in a launch of their own
+// batches of at least this many echo cells (frames x pulses x range cells; 32 frames of 256 x 4096)
+// update the final rows of range alignment in a launch of their own
+constexpr long long kRaSplitCells = 1LL << 25;
 
 // Per-frame scalar decisions written by the device (one record per frame).
 struct FrameDecisions {

@@ -51,6 +51,10 @@ CASES = [
     ("c5", 5, {}),                          # 1024 x 16384, 1000 scatterers, 7 km/s
 ]
 
+# Batches of copies of a case's frame large enough (frames x pulses x range cells >= 2^25) for
+# the alignment update's two-grid launch; every copy must equal the single frame bit for bit.
+COPIES = {"c2": 32, "c5": 2}
+
 # Image bar per case. At 1024 pulses x 16384 cells the FP32 lags stay within the 1e-3-cell bar
 # (2.2e-4 measured) and every integer decision is identical, but those lag differences and the
 # FP32 Doppler start (3.7e-3 rad over 1024 pulses) are amplified by the 120-iteration fixed point
@@ -69,6 +73,13 @@ def test_pipeline_parity(name, cid, over):
     rep = compare(img, t, img_o, tr, M, img_rel_l2=IMG_BAR.get(name, IMG_REL_L2))
     print(rep)
     assert rep.ok(), str(rep)
+    n = COPIES.get(name)
+    if n:
+        print(f"[case] {name}: {n} copies in one batch", flush=True)
+        imgs, ts = _run_device(np.stack([echo] * n), np.stack([vel] * n), radar, M, N, isar.Options(), frames=n)
+        for k in range(n):
+            assert np.array_equal(imgs[k], img), f"copy {k} differs from the single frame"
+            assert np.array_equal(ts[k].phi, t.phi) and ts[k].cut_offset == t.cut_offset
 
 
 @pytest.mark.parametrize("cid,seeds", [(1, range(500, 548)), (2, range(600, 616)), (4, range(700, 716))],
