@@ -1,7 +1,7 @@
 // isar_oracle.cpp — FP64 CPU restatement of the reference imaging chain (test infrastructure).
 //
-// This file is the checker for the HIP pipeline and the CPU baseline timed by bench.py. It is
-// never linked into the product library. Each function cites the reference lines it restates;
+// The HIP pipeline is tested against this oracle, and bench.py times it as the CPU baseline. It
+// is never linked into the product library. Each function cites the reference lines it restates;
 // paths are relative to the reference repository root.
 //
 // Deviations from the reference (each one resolves undefined or NaN-producing behaviour; all

@@ -1,7 +1,7 @@
 /*
  * isar_oracle.h — FP64 CPU restatement of the reference ISAR range-Doppler imaging chain.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker that the HIP pipeline is compared
+ * TEST INFRASTRUCTURE ONLY. This library is the oracle that the HIP pipeline is compared
  * against, and the CPU baseline that bench.py times. Product code never links, loads or calls it.
  *
  * Every stage follows the reference source literally (same stage order, same data flow, same

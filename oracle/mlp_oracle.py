@@ -1,6 +1,6 @@
 """FP64 restatement of the warm-start network (test infrastructure only).
 
-Only the tests and bench.py's CPU-baseline leg use this module, as a checker. The product path
+Only the tests and bench.py's CPU-baseline leg use this module, as the oracle. The product path
 runs the HIP kernels of isar_imaging_cuda_amd/csrc/kernels/nn.hip through include/isar_learned.h.
 
 The network is a stack of dense layers with ReLU after every layer but the last, with parameters

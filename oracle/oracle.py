@@ -1,7 +1,7 @@
 """ctypes binding of the FP64 oracle (test infrastructure only).
 
-Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, as the
-checker. The product package never imports it.
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module, to
+compare against or to time it. The product package never imports it.
 """
 from __future__ import annotations
 
