@@ -1,4 +1,4 @@
-"""Pins the FP64 oracle (the checker of the HIP path) against closed forms, golden vectors made
+"""Pins the FP64 oracle (what the HIP path is tested against) against closed forms, golden vectors made
 by independent libraries (tests/golden/make_golden.py) and known-answer constructions."""
 from __future__ import annotations
 
