@@ -357,9 +357,15 @@ __global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) voi
     for (int e = 0; e < kE; ++e) acc[e] = 0.0f;
     float2 x[kE];
     const FftTwiddles<N, kE> tw = fft_twiddles<N, kE>(tid);
-    // column sums of |profile| over the final rows of this chunk, accumulated in place (each
-    // location is read back only by the thread that wrote it)
+    // column sums of |profile| over the final rows of this chunk: kept in registers by the
+    // final-row grid up to 4096 cells (above, its workgroup size leaves no room) and stored once,
+    // otherwise accumulated in place (each location is read back only by the thread that wrote
+    // it); the same additions in the same order either way
+    constexpr bool kArpInRegs = PART == kUpdFinal && N <= 4096;
     float* ap = J.arp_part + (static_cast<size_t>(f) * (M / 2) + r0 / 2) * N;
+    float arp[kArpInRegs ? kE : 1];
+#pragma unroll
+    for (int e = 0; e < (kArpInRegs ? kE : 1); ++e) arp[e] = 0.0f;
     for (int m = r0; m < r1; ++m) {
         const double L = J.lagsum[static_cast<size_t>(f) * M + m] + static_cast<double>(lag);
         load_row_shifted<N>(x, J, f, m, L, tid);
@@ -375,12 +381,18 @@ __global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) voi
                 const int pos = out_pos<N>(t, e);
                 st[pos] = v;
                 const float av = cabs_hw(v);
-                ap[pos] = (m == r0 ? 0.0f : ap[pos]) + av;
+                if constexpr (kArpInRegs) arp[e] += av;
+                else ap[pos] = (m == r0 ? 0.0f : ap[pos]) + av;
                 mx = fmaxf(mx, av);
             }
         }
     }
     if (final_rows) {
+        if constexpr (kArpInRegs) {
+            const int t = opaque(tid);
+#pragma unroll
+            for (int e = 0; e < kE; ++e) ap[out_pos<N>(t, e)] = arp[e];
+        }
         float* red = reinterpret_cast<float*>(lds + lds_padded_size(N));
         mx = block_max<N / kE>(mx, red);
         if (tid == 0) J.max_part[static_cast<size_t>(f) * (M / 2) + r0 / 2] = mx;
