@@ -53,7 +53,7 @@ CASES = [
 
 # Batches of copies of a case's frame large enough (frames x pulses x range cells >= 2^25) for
 # the alignment update's two-grid launch; every copy must equal the single frame bit for bit.
-COPIES = {"c2": 32, "c5": 2}
+COPIES = {"c1": 256, "c4": 64, "c2": 32, "c5": 2}
 
 # Image bar per case. At 1024 pulses x 16384 cells the FP32 lags stay within the 1e-3-cell bar
 # (2.2e-4 measured) and every integer decision is identical, but those lag differences and the
