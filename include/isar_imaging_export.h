@@ -8,8 +8,10 @@
  * Differences a caller can observe:
  *   - nothing calls exit(): int functions return EXIT_FAILURE and void functions leave an error
  *     message in isar_last_error() (isar_imaging_ext.h) instead of terminating;
- *   - dataExtracting / isarMainSingle with data_type IFDS return or record an error (the IFDS
- *     pulse-compression ingest is not provided; STRETCH is).
+ *   - dataExtracting checks its arguments against imagingMemInit and dataParsing (data type,
+ *     frame length, window length, vector sizes) and returns EXIT_FAILURE on a mismatch, where
+ *     the reference would overrun a buffer. Both ingests are provided: STRETCH dechirped frames
+ *     and IFDS raw echoes, whose pulse compression runs on the GPU.
  */
 #ifndef ISAR_IMAGING_EXPORT_H_
 #define ISAR_IMAGING_EXPORT_H_

@@ -79,7 +79,7 @@ typedef struct isar_frame_trace {
     int32_t status;
     float   center_thr;
     float   pa_threshold;
-    float*  ra_lag;           /* [M-1] level-major, pair-minor */
+    double* ra_lag;           /* [M-1] level-major, pair-minor */
     int32_t* ra_imax;         /* [M-1] */
     float*  pa_colmax;        /* [W] */
     float*  pa_colE;          /* [W] */
@@ -183,6 +183,13 @@ ISAR_API int isar_set_graph_mode(isar_context* ctx, int on);
  * choice changes the summation order of the group energies, not the algorithm. */
 ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows);
 ISAR_API int isar_get_ra_chunk(isar_context* ctx);
+/* Grids of the range-alignment update per merge level: 0 (default) by batch size, one grid below
+ * 2^25 echo cells per batch and two from there (the rows of pair 0, which move for the last time
+ * and carry the aligned-profile output, apart from the other pairs); 1 always one grid; 2 always
+ * two. All give the same results bit for bit; the setting exists so that both launch forms can
+ * be exercised on small frames. */
+ISAR_API int isar_set_ra_split(isar_context* ctx, int mode);
+ISAR_API int isar_get_ra_split(isar_context* ctx);
 /* Autotuner: times every range-alignment chunk size on these frames (`repeats` runs each after
  * one warm-up, on `stream`), keeps the fastest for the context and returns it in *chosen; then
  * times the autofocus forms with that chunk size and keeps the fastest (isar_get_autofocus_mode).

@@ -37,12 +37,13 @@ class _Options(C.Structure):
 
 _fp = C.POINTER(C.c_float)
 _ip = C.POINTER(C.c_int32)
+_dp = C.POINTER(C.c_double)
 
 
 class _Trace(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("n_pairs", "center_applied", "center_shift", "idx1_len", "idx2_len",
                                           "diff_imin", "cut_offset", "tgt_num", "unit2_num", "status")] + \
-               [("center_thr", C.c_float), ("pa_threshold", C.c_float), ("ra_lag", _fp), ("ra_imax", _ip),
+               [("center_thr", C.c_float), ("pa_threshold", C.c_float), ("ra_lag", _dp), ("ra_imax", _ip),
                 ("pa_colmax", _fp), ("pa_colE", _fp), ("pa_colH", _fp), ("select_bin", _ip), ("phi0", _fp),
                 ("phi", _fp)]
 
@@ -285,6 +286,13 @@ class Imager:
         L.isar_set_ra_chunk.argtypes = [C.c_void_p, C.c_int]
         _check(L.isar_set_ra_chunk(self._h, int(rows)), "isar_set_ra_chunk")
 
+    def set_ra_split(self, mode: int) -> None:
+        """Alignment update grids per merge level: 0 by batch size, 1 one grid, 2 two grids (the
+        rows of pair 0 apart); bit-identical results."""
+        L = lib()
+        L.isar_set_ra_split.argtypes = [C.c_void_p, C.c_int]
+        _check(L.isar_set_ra_split(self._h, int(mode)), "isar_set_ra_split")
+
     def autotune(self, echo_ptr: int, velocity_ptr: int | None, img_ptr: int, frames: int,
                  options: Options | None = None, repeats: int = 3, stream: int | None = None) -> int:
         """Time every range-alignment chunk size on these frames, keep the fastest and return it."""
@@ -310,14 +318,14 @@ class Imager:
 
     def trace(self, frame: int = 0) -> FrameTrace:
         M, W = self.params.echo_num, self.params.range_num_cut
-        lag = np.zeros(M - 1, np.float32)
+        lag = np.zeros(M - 1, np.float64)
         imax = np.zeros(M - 1, np.int32)
         cm, ce, ch = (np.zeros(W, np.float32) for _ in range(3))
         sel = np.full(W, -1, np.int32)
         p0 = np.zeros(2 * M, np.float32)
         p1 = np.zeros(2 * M, np.float32)
         t = _Trace()
-        t.ra_lag = lag.ctypes.data_as(_fp)
+        t.ra_lag = lag.ctypes.data_as(_dp)
         t.ra_imax = imax.ctypes.data_as(_ip)
         t.pa_colmax = cm.ctypes.data_as(_fp)
         t.pa_colE = ce.ctypes.data_as(_fp)

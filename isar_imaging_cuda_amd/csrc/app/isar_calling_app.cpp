@@ -10,6 +10,8 @@
 // with dump_dir the first window's intermediate arrays as the reference's text dumps
 // (dataW.dat ... final.dat).
 // n_images = 0 only parses the directory (no GPU call) and prints the decoded headers.
+// n_images = -1 checks that dataExtracting refuses arguments that do not match imagingMemInit
+// (frame length, data type, a too-small output vector) instead of overrunning a buffer.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -49,7 +51,7 @@ int main(int argc, char** argv) {
     vec1D_COM_FLT dataW;
     vec1D_FLT img;
 
-    if (n_images > 0 && gpuDevInit() != EXIT_SUCCESS) {
+    if (n_images != 0 && gpuDevInit() != EXIT_SUCCESS) {
         std::fprintf(stderr, "gpuDevInit: %s\n", isar_last_error());
         return 3;
     }
@@ -69,6 +71,23 @@ int main(int argc, char** argv) {
     if (img.size() != static_cast<size_t>(window_len) * 512) {
         std::fprintf(stderr, "imagingMemInit: %s\n", isar_last_error());
         return 5;
+    }
+    if (n_images < 0) {
+        int refused = 0;
+        auto check = [&](const char* what, int fl, int dt, vec1D_COM_FLT* w) {
+            const int rc = dataExtracting(&dataWFileSn, &dataNOut, &turnAngleOut, w, dataN, turnAngle, fl, frame_num,
+                                          sampling_stride, window_head, window_len, imaging_stride, dt);
+            std::printf("%s: rc %d (%s)\n", what, rc, rc == EXIT_SUCCESS ? "" : isar_last_error());
+            refused += rc != EXIT_SUCCESS;
+        };
+        vec1D_COM_FLT small(dataW.size() / 2);
+        check("frame_len + 4", frame_len + 4, data_type, &dataW);
+        check("other data_type", frame_len, data_type == 2 ? 1 : 2, &dataW);
+        if (data_type == 2) check("dataW half size", frame_len, data_type, &small);
+        else ++refused;
+        check("matching", frame_len, data_type, &dataW);   // must still work afterwards
+        imagingMemDest(data_type, if_hpc, if_hrrp);
+        return refused == 3 ? 0 : 9;
     }
     for (int i = 0; i < n_images; ++i) {
         auto t1 = std::chrono::high_resolution_clock::now();

@@ -1,10 +1,10 @@
 // emu_driver.cpp — runs the kernel sources on the CPU (hip_emu.hpp) through the same C ABI as
 // the device library. Test infrastructure: built with AddressSanitizer by the CPU test suite.
 //
-// usage: isar_emu_driver <input.bin> <output.bin> [autofocus_mode]
+// usage: isar_emu_driver <input.bin> <output.bin> [autofocus_mode [ra_split]]
 // input : int32 M, N, W, K; int64 band_width, fc; double Tp; int64 Fs; isar_options;
 //         echo K*M*N complex float; velocity K*M double; [phi0 K*M complex float if warm_start]
-// output: img K*M*W float; then per frame: int32[10] scalars, float[2], lag[M-1], imax[M-1],
+// output: img K*M*W float; then per frame: int32[10] scalars, float[2], double lag[M-1], imax[M-1],
 //         colmax[W], colE[W], colH[W], select_bin[W], phi0[2M], phi[2M]
 #include <cstdio>
 #include <cstdlib>
@@ -13,8 +13,8 @@
 #include "isar_imaging_ext.h"
 
 int main(int argc, char** argv) {
-    if (argc != 3 && argc != 4) {
-        std::fprintf(stderr, "usage: %s input.bin output.bin [autofocus_mode]\n", argv[0]);
+    if (argc < 3 || argc > 5) {
+        std::fprintf(stderr, "usage: %s input.bin output.bin [autofocus_mode [ra_split]]\n", argv[0]);
         return 2;
     }
     FILE* in = std::fopen(argv[1], "rb");
@@ -40,7 +40,11 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "create: %s\n", isar_last_error());
         return 6;
     }
-    if (argc == 4 && isar_set_autofocus_mode(ctx, std::atoi(argv[3])) != 0) {
+    if (argc == 5 && isar_set_ra_split(ctx, std::atoi(argv[4])) != 0) {
+        std::fprintf(stderr, "ra split: %s\n", isar_last_error());
+        return 6;
+    }
+    if (argc >= 4 && isar_set_autofocus_mode(ctx, std::atoi(argv[3])) != 0) {
         std::fprintf(stderr, "autofocus mode: %s\n", isar_last_error());
         return 6;
     }
@@ -53,7 +57,8 @@ int main(int argc, char** argv) {
     if (!out) return 8;
     std::fwrite(img.data(), sizeof(float), img.size(), out);
     for (int f = 0; f < K; ++f) {
-        std::vector<float> lag(M - 1), cm(W), ce(W), ch(W), p0(2 * M), p1(2 * M);
+        std::vector<double> lag(M - 1);
+        std::vector<float> cm(W), ce(W), ch(W), p0(2 * M), p1(2 * M);
         std::vector<int32_t> imax(M - 1), sel(W, -1);
         isar_frame_trace t{};
         t.ra_lag = lag.data();
@@ -70,7 +75,7 @@ int main(int argc, char** argv) {
         float fs[2] = {t.center_thr, t.pa_threshold};
         std::fwrite(sc, sizeof(sc), 1, out);
         std::fwrite(fs, sizeof(fs), 1, out);
-        std::fwrite(lag.data(), sizeof(float), lag.size(), out);
+        std::fwrite(lag.data(), sizeof(double), lag.size(), out);
         std::fwrite(imax.data(), sizeof(int32_t), imax.size(), out);
         std::fwrite(cm.data(), sizeof(float), W, out);
         std::fwrite(ce.data(), sizeof(float), W, out);

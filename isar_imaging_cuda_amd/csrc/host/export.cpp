@@ -49,6 +49,9 @@ struct Session {
     int16_t* d_iq = nullptr;
     double* d_pc_velocity = nullptr;
     int iq_len = 0;
+    // what imagingMemInit sized the buffers for; dataExtracting must match it
+    int init_data_type = 0;
+    int init_frame_len = 0;
 };
 Session g;
 
@@ -177,6 +180,8 @@ void free_buffers() {
     g.d_pc_velocity = nullptr;
     g.iq_len = 0;
     g.active = -1;
+    g.init_data_type = 0;
+    g.init_frame_len = 0;
 }
 
 }  // namespace
@@ -256,6 +261,8 @@ DLL_EXPORT_API void imagingMemInit(vec1D_FLT* img, vec1D_INT* dataWFileSn, vec1D
     }
     g.paras = p;
     g.h_velocity.assign(M, 0.0);
+    g.init_data_type = data_type;
+    g.init_frame_len = frame_len;
 }
 
 DLL_EXPORT_API int dataExtracting(vec1D_INT* dataWFileSn, vec1D_DBL* dataNOut, vec1D_FLT* turnAngleOut, vec1D_COM_FLT* dataW,
@@ -263,10 +270,22 @@ DLL_EXPORT_API int dataExtracting(vec1D_INT* dataWFileSn, vec1D_DBL* dataNOut, v
                                   const int& frame_num, const int& sampling_stride, const int& window_head,
                                   const int& window_len, int& imaging_stride, const int& data_type) {
     if (!g.ctx) return err("dataExtracting before imagingMemInit"), EXIT_FAILURE;
+    if (!dataWFileSn || !dataNOut || !turnAngleOut || !dataW) return err("null output vector"), EXIT_FAILURE;
     if (data_type != kStretch && data_type != kIfds) return err("unknown data type"), EXIT_FAILURE;
-    if (data_type == kIfds && !g.d_iq) return err("IFDS data after an imagingMemInit for STRETCH"), EXIT_FAILURE;
+    // the buffers were sized by imagingMemInit for one data type and frame length, and the files
+    // hold frames of the length dataParsing read: anything else would overrun them
+    if (data_type != g.init_data_type) return err("data_type differs from imagingMemInit"), EXIT_FAILURE;
+    if (frame_len != g.init_frame_len || frame_len != g.frame_len)
+        return err("frame_len differs from imagingMemInit or from the files"), EXIT_FAILURE;
     const int M = g.paras.echo_num, N = g.paras.range_num;
     if (window_len != M) return err("window_len differs from imagingMemInit"), EXIT_FAILURE;
+    const size_t total_frames = static_cast<size_t>(frame_num) * g.files.size();
+    if (frame_num != g.frame_num || dataN.size() < 4 * total_frames || turnAngle.size() < total_frames)
+        return err("dataN / turnAngle / frame_num do not match dataParsing"), EXIT_FAILURE;
+    if (dataWFileSn->size() < static_cast<size_t>(M) || dataNOut->size() < 4 * static_cast<size_t>(M) ||
+        turnAngleOut->size() < static_cast<size_t>(M) ||
+        (data_type == kStretch && dataW->size() < static_cast<size_t>(M) * N))
+        return err("output vectors smaller than imagingMemInit sized them"), EXIT_FAILURE;
     // uniformSampling (common.cu:1165-1185)
     const int total = frame_num * static_cast<int>(g.files.size());
     for (int i = 0; i < M; ++i) {

@@ -81,7 +81,7 @@ size_t workspace_bytes(const isar_radar_params& p, int K) {
     per += M * N * sizeof(float2);              // pre / st
     per += 2 * M * N * sizeof(float);           // egrp, epart
     per += nchunks * N * sizeof(float) + nchunks * sizeof(float);
-    per += M * sizeof(double) + 2 * M * sizeof(float);          // lagsum, lags, imax
+    per += 2 * M * sizeof(double) + M * sizeof(float);          // lagsum, lags, imax
     per += sizeof(FrameDecisions);
     per += 4 * M * W * sizeof(float2);          // x, stm, czt, su
     per += M * sizeof(float2) * 3;              // rowdop, phi0, phi
@@ -168,7 +168,7 @@ int Context::init(const isar_radar_params& params, int frames) {
     J.arp_part = static_cast<float*>(alloc(K * nchunks * N * sizeof(float)));
     J.max_part = static_cast<float*>(alloc(K * nchunks * sizeof(float)));
     J.lagsum = static_cast<double*>(alloc(K * M * sizeof(double)));
-    J.lags = static_cast<float*>(alloc(K * M * sizeof(float)));
+    J.lags = static_cast<double*>(alloc(K * M * sizeof(double)));
     J.imax = static_cast<int32_t*>(alloc(K * M * sizeof(int32_t)));
     J.dec = static_cast<FrameDecisions*>(alloc(K * sizeof(FrameDecisions)));
     J.x = static_cast<float2*>(alloc(K * M * W * sizeof(float2)));
@@ -243,11 +243,12 @@ int Context::run_frames(int frames, const void* echo, const double* velocity, co
     k.opt = opt;
     k.ra_chunk = ra_chunk;
     k.af_mode = af_mode;
+    k.ra_split = ra_split;
     k.hrrp = hrrp_out;
     const GraphKey& g0 = graph_key;
     const bool same = graph_exec && k.frames == g0.frames && k.echo == g0.echo && k.velocity == g0.velocity &&
                       k.frame_row == g0.frame_row && k.phi0 == g0.phi0 && k.img == g0.img &&
-                      k.frame_img == g0.frame_img && k.ra_chunk == g0.ra_chunk && k.af_mode == g0.af_mode &&
+                      k.frame_img == g0.frame_img && k.ra_chunk == g0.ra_chunk && k.af_mode == g0.af_mode && k.ra_split == g0.ra_split &&
                       k.hrrp == g0.hrrp && std::memcmp(&k.opt, &g0.opt, sizeof(isar_options)) == 0;
     if (!same) {
         // capture the launches on an internal stream; the graph is then launched on the caller's
@@ -313,6 +314,7 @@ int Context::enqueue(int frames, const void* echo, const double* velocity, const
     J.frame_row = frame_row;
     J.frame_img = frame_img;
     J.ra_chunk = ra_chunk;
+    J.ra_split = ra_split;
     J.phi0_in = static_cast<const float2*>(phi0);
     J.img = img;
     J.hrrp = opt.if_hrrp && mode != ISAR_RUN_BACK ? hrrp_out : nullptr;
@@ -446,7 +448,7 @@ int Context::trace(int frame, isar_frame_trace* out) {
     auto cp = [&](void* dst, const void* src, size_t bytes) {
         if (dst && e == hipSuccess) e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
     };
-    cp(out->ra_lag, J.lags + frame * (M - 1), (M - 1) * sizeof(float));
+    cp(out->ra_lag, J.lags + frame * (M - 1), (M - 1) * sizeof(double));
     cp(out->ra_imax, J.imax + frame * (M - 1), (M - 1) * sizeof(int32_t));
     cp(out->pa_colmax, J.colmax + frame * W, W * sizeof(float));
     cp(out->pa_colE, J.colE + frame * W, W * sizeof(float));
@@ -607,6 +609,16 @@ ISAR_API int isar_set_ra_chunk(isar_context* ctx, int rows) {
 }
 
 ISAR_API int isar_get_ra_chunk(isar_context* ctx) { return ctx ? ctx->ra_chunk : 0; }
+
+ISAR_API int isar_set_ra_split(isar_context* ctx, int mode) {
+    if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");
+    if (mode < 0 || mode > 2)
+        return isar::fail(ISAR_ERR_INVALID, "ra split must be 0 (by batch size), 1 (one grid) or 2 (two grids)");
+    ctx->ra_split = mode;
+    return ISAR_OK;
+}
+
+ISAR_API int isar_get_ra_split(isar_context* ctx) { return ctx ? ctx->ra_split : -1; }
 
 ISAR_API int isar_set_autofocus_mode(isar_context* ctx, int mode) {
     if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");

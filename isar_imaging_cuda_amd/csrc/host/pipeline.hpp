@@ -67,7 +67,7 @@ struct Context {
         float* img = nullptr;
         const long long* frame_img = nullptr;
         isar_options opt{};
-        int ra_chunk = 0, af_mode = 0;
+        int ra_chunk = 0, af_mode = 0, ra_split = 0;
         const float2* hrrp = nullptr;
     };
     bool graph_mode = false;
@@ -87,6 +87,7 @@ struct Context {
     float tuned_ms = 0.0f;                // time of one run in the configuration the autotuner kept
     float mode_ms[4] = {0, 0, 0, 0};      // time of one run per autofocus form (index: isar_autofocus_mode)
     int ra_chunk = kRaChunkRowsDefault;   // tunable: rows per workgroup in the alignment update
+    int ra_split = 0;                     // alignment update grids per level (isar_set_ra_split)
     // times every range-alignment chunk size on these frames and keeps the fastest (synchronous)
     int autotune(int frames, const void* echo, const double* velocity, const long long* frame_row, float* img,
                  const long long* frame_img, const isar_options& opt, hipStream_t s, int repeats, int* chosen);

@@ -104,13 +104,12 @@ __device__ __forceinline__ void group_energy(float (&a)[kE], const Job& J, int f
     }
 }
 
-// Correlation of two non-negative profiles given as energies (first-pass layout): argmax of |c|
-// after ifftshift (lowest index on ties) and the parabolic lag (range_alignment.cu:45-61,
-// 108-168). The real correlation comes from one complex FFT of gA + j gB. Thread 0 records the
-// lag and argmax; every thread gets the lag.
+// Correlation of two non-negative profiles given as energies (first-pass layout): the argmax of
+// |c| after ifftshift, lowest index on ties (range_alignment.cu:45-61, 108-168). The real
+// correlation comes from one complex FFT of gA + j gB. Every thread gets the argmax.
 template <int N>
-__device__ float correlate_pair(const float (&a)[kE], const float (&b)[kE], float2* lds, float* redv, int* redi,
-                                int tid, const FftTwiddles<N, kE>& tw, float* lag_out, int32_t* imax_out) {
+__device__ int correlation_argmax(const float (&a)[kE], const float (&b)[kE], float2* lds, float* redv, int* redi,
+                                  int tid, const FftTwiddles<N, kE>& tw) {
     constexpr int NT = N / kE;
     float2 x[kE];
 #pragma unroll
@@ -137,26 +136,78 @@ __device__ float correlate_pair(const float (&a)[kE], const float (&b)[kE], floa
         const float v = fabsf(x[e].x);
         if (v > best || (v == best && i < bi)) { best = v; bi = i; }
     }
-    float* cl = reinterpret_cast<float*>(lds);
-#pragma unroll
-    for (int e = 0; e < kE; ++e) cl[(out_pos<N>(tid, e) + N / 2) & (N - 1)] = x[e].x;
+    // the transform's last LDS reads precede the reduction's barriers
     block_argmax<NT>(best, bi, redv, redi);
-    if (tid == 0) {
-        float delta = 0.0f;
-        if (bi > 0 && bi < N - 1) {
-            const float f1 = cl[bi - 1], f2 = cl[bi], f3 = cl[bi + 1];
-            const float fa = (f1 + f3 - 2.0f * f2) / 2.0f, fb = (f3 - f1) / 2.0f;
-            if (fa != 0.0f) delta = -fb / (2.0f * fa);
-            if (!(delta == delta) || fabsf(delta) > 1e30f) delta = 0.0f;
-        }
-        // integer part first: bi + delta rounded at bi's magnitude would lose ~N/2^24 cells
-        const float lag = static_cast<float>(bi - N / 2) + delta;
-        *lag_out = lag;
-        *imax_out = bi;
-        redv[32] = lag;
+    return bi;
+}
+
+// Parabolic lag around the argmax bi (binomialFixDevice, range_alignment.cu:121-131). The three
+// samples c[bi-1], c[bi], c[bi+1] are formed again as direct sums sum_n gA[n] gB[n - l] in FP64,
+// not read from the FP32 transform: near a broad peak the second
+// difference f1 + f3 - 2 f2 cancels most of its digits, and the transform's rounding alone moved
+// the vertex by up to 2.4e-4 cell at 1024 x 16384, which 120 autofocus iterations amplify into a
+// 1e-3 image difference. The vertex and the lag are FP64. a, b: the two energy profiles, element
+// e of thread tid at range cell pos(tid, e), where pos is the first-pass layout (OUT = false) or
+// the last-pass layout (OUT = true). Must follow correlation_argmax (its barriers end the LDS
+// use). Thread 0 records the lag and argmax; every thread gets the lag.
+template <int N, bool OUT = false>
+__device__ double parabola_lag(const float (&a)[kE], const float (&b)[kE], int bi, float2* lds, int tid,
+                               double* lag_out, int32_t* imax_out) {
+    constexpr int NT = N / kE;
+    // cl[i] = c[i - N/2] and c[l] = sum_n gA[n] gB[n - l] (mod N): the samples at bi - 1, bi,
+    // bi + 1 are lags l0 - 1, l0, l0 + 1 with l0 = bi - N/2. gB goes to LDS by range cell (N
+    // doubles fit the transform's padded N float2); each thread sums over its own cells. The
+    // per-wave sums follow gB in LDS and are added in wave order, so the result does not depend
+    // on scheduling.
+    // The profiles are the FP64 square roots of the FP32 energies (the hardware FP32 root is
+    // off by up to 1 ulp, which the vertex would see).
+    double* gb = reinterpret_cast<double*>(lds);
+    double* red = gb + N;
+    auto pos = [&](int e) { return OUT ? out_pos<N>(tid, e) : in_pos<N>(tid, e); };
+#pragma unroll
+    for (int e = 0; e < kE; ++e) gb[pos(e)] = sqrt(static_cast<double>(b[e]));
+    __syncthreads();
+    const int l0 = bi - N / 2;
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+    for (int e = 0; e < kE; ++e) {
+        const int n = pos(e);
+        const double ga = sqrt(static_cast<double>(a[e]));
+        s1 = fma(ga, gb[(n - l0 + 1) & (N - 1)], s1);
+        s2 = fma(ga, gb[(n - l0) & (N - 1)], s2);
+        s3 = fma(ga, gb[(n - l0 - 1) & (N - 1)], s3);
+    }
+    s1 = wave_sum_d(s1);
+    s2 = wave_sum_d(s2);
+    s3 = wave_sum_d(s3);
+    const int w = tid >> 6;
+    if ((tid & 63) == 0) {
+        red[3 * w] = s1;
+        red[3 * w + 1] = s2;
+        red[3 * w + 2] = s3;
     }
     __syncthreads();
-    const float lag = redv[32];
+    if (tid == 0) {
+        double f1 = 0.0, f2 = 0.0, f3 = 0.0;
+#pragma unroll
+        for (int k = 0; k < (NT + 63) / 64; ++k) {
+            f1 += red[3 * k];
+            f2 += red[3 * k + 1];
+            f3 += red[3 * k + 2];
+        }
+        double delta = 0.0;
+        if (bi > 0 && bi < N - 1) {   // Q9a: no refinement at the row edge
+            const double fa = (f1 + f3 - 2.0 * f2) / 2.0, fb = (f3 - f1) / 2.0;
+            if (fa != 0.0) delta = -fb / (2.0 * fa);
+            if (!(delta == delta) || fabs(delta) > 1e30) delta = 0.0;
+        }
+        const double lag = static_cast<double>(l0) + delta;
+        *lag_out = lag;
+        *imax_out = bi;
+        red[0] = lag;
+    }
+    __syncthreads();
+    const double lag = red[0];
     __syncthreads();
     return lag;
 }
@@ -262,6 +313,17 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
         for (int e = 0; e < kE; ++e) s0b[in_pos<N>(t, e)] = x[e];
     }
     fft<N, kE, +1>(x, lds, tid, tw);
+    // Above 4096 cells the two energy rows do not stay in registers across the correlation's
+    // transforms: row 2p's is in its merged-energy row already, and row 2p+1's goes to that
+    // row's energy slot, which no group at stride >= 2 uses (group heads are even rows). Each
+    // thread reads back only what it wrote.
+    constexpr bool kStash = N > 4096;
+    float* eb_row = J.egrp + (static_cast<size_t>(f) * M + rb) * N;
+    if constexpr (kStash) {
+        const int t = opaque(tid);
+#pragma unroll
+        for (int e = 0; e < kE; ++e) eb_row[out_pos<N>(t, e)] = cabs2(x[e]) * inv2;
+    }
     // the correlation takes the energies in the first-pass layout
 #pragma unroll
     for (int e = 0; e < kE; ++e) x[e] = make_float2(ea[e], cabs2(x[e]) * inv2);
@@ -270,9 +332,21 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
 #pragma unroll
     for (int e = 0; e < kE; ++e) { ta[e] = x[e].x; tb[e] = x[e].y; }
     const size_t idx = static_cast<size_t>(f) * (M - 1) + p;
-    const float lag = correlate_pair<N>(ta, tb, lds, redv, redi, tid, tw, J.lags + idx, J.imax + idx);
+    const int bi = correlation_argmax<N>(ta, tb, lds, redv, redi, tid, tw);
+    double lag;
+    if constexpr (kStash) {
+        const int t = opaque(tid);
+#pragma unroll
+        for (int e = 0; e < kE; ++e) {
+            ta[e] = dst[out_pos<N>(t, e)];
+            tb[e] = eb_row[out_pos<N>(t, e)];
+        }
+        lag = parabola_lag<N, true>(ta, tb, bi, lds, tid, J.lags + idx, J.imax + idx);
+    } else {
+        lag = parabola_lag<N>(ta, tb, bi, lds, tid, J.lags + idx, J.imax + idx);
+    }
     // shift row 2p+1 by the lag and merge (range_alignment.cu:66-69)
-    load_row_shifted<N>(x, J, f, rb, static_cast<double>(lag), tid);
+    load_row_shifted<N>(x, J, f, rb, lag, tid);
     fft<N, kE, +1>(x, lds, tid, tw);
     {
         const int t = opaque(tid);
@@ -281,7 +355,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
     }
     if (tid == 0) {
         J.lagsum[static_cast<size_t>(f) * M + ra] = 0.0;
-        J.lagsum[static_cast<size_t>(f) * M + rb] = static_cast<double>(lag);
+        J.lagsum[static_cast<size_t>(f) * M + rb] = lag;
     }
     if (final_rows) {   // row 1 never moves again either
         const int t = opaque(tid);
@@ -322,7 +396,17 @@ __global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base)
     if (unpaired) return;
     group_energy<N>(b, J, f, gA + 1, s, tid);
     const size_t idx = static_cast<size_t>(f) * (M - 1) + pair_base + p;
-    (void)correlate_pair<N>(a, b, lds, redv, redi, tid, tw, J.lags + idx, J.imax + idx);
+    const int bi = correlation_argmax<N>(a, b, lds, redv, redi, tid, tw);
+    // read both energies again rather than hold them across the transforms (the first from the
+    // row this thread just wrote)
+    {
+        const float* src = J.egrp + (static_cast<size_t>(f) * M + static_cast<size_t>(gA) * s) * N;
+        const int t = opaque(tid);
+#pragma unroll
+        for (int e = 0; e < kE; ++e) a[e] = src[in_pos<N>(t, e)];
+    }
+    group_energy<N>(b, J, f, gA + 1, s, opaque(tid));
+    (void)parabola_lag<N>(a, b, bi, lds, tid, J.lags + idx, J.imax + idx);
 }
 
 // Shift the rows of the second group of pair p by the pair's lag (range_alignment.cu:66-69)
@@ -347,7 +431,7 @@ __global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) voi
     const int r0 = (2 * p + 1) * s + i * R;
     const int r1 = min(r0 + R, gend);
     if (r0 >= r1) return;
-    const float lag = J.lags[static_cast<size_t>(f) * (M - 1) + pair_base + p];
+    const double lag = J.lags[static_cast<size_t>(f) * (M - 1) + pair_base + p];
     const float inv = 1.0f / (static_cast<float>(N) * static_cast<float>(N));
     const float invN = 1.0f / N;
     const bool final_rows = PART == kUpdFinal || (PART == kUpdAll && p == 0);
@@ -367,7 +451,7 @@ __global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) voi
 #pragma unroll
     for (int e = 0; e < (kArpInRegs ? kE : 1); ++e) arp[e] = 0.0f;
     for (int m = r0; m < r1; ++m) {
-        const double L = J.lagsum[static_cast<size_t>(f) * M + m] + static_cast<double>(lag);
+        const double L = J.lagsum[static_cast<size_t>(f) * M + m] + lag;
         load_row_shifted<N>(x, J, f, m, L, tid);
         fft<N, kE, +1>(x, lds, tid, tw);
 #pragma unroll
@@ -398,7 +482,7 @@ __global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) voi
         if (tid == 0) J.max_part[static_cast<size_t>(f) * (M / 2) + r0 / 2] = mx;
     }
     __syncthreads();
-    if (tid < r1 - r0) J.lagsum[static_cast<size_t>(f) * M + r0 + tid] += static_cast<double>(lag);
+    if (tid < r1 - r0) J.lagsum[static_cast<size_t>(f) * M + r0 + tid] += lag;
     float* dst = J.epart + (static_cast<size_t>(f) * M + r0) * N;
 #pragma unroll
     for (int e = 0; e < kE; ++e) dst[out_pos<N>(tid, e)] = acc[e];
@@ -467,7 +551,8 @@ void launch_ra_corr(const Job& J, int stride, int pair_base, int nblocks, hipStr
 void launch_ra_upd(const Job& J, int stride, int pair_base, int nblocks, hipStream_t s) {
     // nblocks = pairs x chunks per pair. Large batches launch pair 0 (final rows) and the other
     // pairs apart; small ones, where the second launch per level costs more than it saves, in one.
-    if (static_cast<long long>(J.K) * J.M * J.N < kRaSplitCells) {
+    const bool split = J.ra_split == 2 || (J.ra_split == 0 && static_cast<long long>(J.K) * J.M * J.N >= kRaSplitCells);
+    if (!split) {
         ISAR_RANGE_SWITCH(J.N, ISAR_LAUNCH((k_ra_upd<NN, kUpdAll>), dim3(nblocks, J.K), dim3(NN / kE), ra_lds<NN>(),
                                            s, J, stride, pair_base));
         return;

@@ -47,6 +47,7 @@ struct Job {
     int if_hpc, if_hrrp, if_mtrc, quirk_q1, iterations;
     int phi0_given;
     int ra_chunk;                 // rows per workgroup in the alignment update (2, 4, 8 or 16)
+    int ra_split;                 // alignment update grids per level: 0 by batch size, 1 one, 2 two
     double hpc_k;                 // HPC phase in half-turns = hpc_k * v * n^2
     double czt_c0, czt_posa;      // alpha_n = 1 - c0 + posa * n
     // inputs
@@ -58,7 +59,7 @@ struct Job {
     // workspaces
     float2* pre;                  // K x M x N: IFDS pre-stage output, then the row spectra S0
     double* lagsum;               // K x M
-    float* lags;                  // K x (M-1)
+    double* lags;                 // K x (M-1)
     int32_t* imax;                // K x (M-1)
     float* egrp;                  // K x M x N
     float* epart;                 // K x M x N

@@ -262,6 +262,35 @@ void orc_hpc(double* s, const double* velocity, const orc_params* p) {
     }
 }
 
+}  // extern "C"
+
+namespace {
+// FP32 radix-2 transform with twiddles rounded once from FP64, for the range_fft32 option.
+thread_local int t_range_fft32 = 0;
+void fft_pow2_f32(cd* x, int n, int dir) {
+    std::vector<std::complex<float>> v(n);
+    for (int i = 0; i < n; ++i) v[i] = std::complex<float>(static_cast<float>(x[i].real()), static_cast<float>(x[i].imag()));
+    for (int i = 1, j = 0; i < n; ++i) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(v[i], v[j]);
+    }
+    for (int len = 2; len <= n; len <<= 1)
+        for (int i = 0; i < n; i += len)
+            for (int k = 0; k < len / 2; ++k) {
+                const double a = dir * 2.0 * kPi * k / len;
+                const std::complex<float> w(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+                const std::complex<float> u = v[i + k], t = v[i + k + len / 2] * w;
+                v[i + k] = u + t;
+                v[i + k + len / 2] = u - t;
+            }
+    for (int i = 0; i < n; ++i) x[i] = cd(v[i].real(), v[i].imag());
+}
+}  // namespace
+
+extern "C" {
+
 void orc_range_alignment(double* s_, const orc_params* p, orc_trace* tr) {
     // range_alignment.cu:4-78, with the group structure generalised to any M (Q10).
     const int M = p->echo_num, N = p->range_num;
@@ -273,6 +302,7 @@ void orc_range_alignment(double* s_, const orc_params* p, orc_trace* tr) {
 
     std::vector<double> P(static_cast<size_t>(M) * N);
     std::vector<double> g(static_cast<size_t>(M) * N);
+    const int fft32 = t_range_fft32;   // read before the parallel regions (thread-local)
     int pair_index = 0;
     for (int s = 1; s < M; s *= 2) {
         // :38-42 profile magnitude of every row: |IFFT(S)| / N
@@ -282,7 +312,8 @@ void orc_range_alignment(double* s_, const orc_params* p, orc_trace* tr) {
 #pragma omp for schedule(static)
             for (int m = 0; m < M; ++m) {
                 std::copy(S + static_cast<size_t>(m) * N, S + static_cast<size_t>(m + 1) * N, row.begin());
-                fft_pow2(row.data(), N, +1);
+                if (fft32) fft_pow2_f32(row.data(), N, +1);
+                else fft_pow2(row.data(), N, +1);
                 for (int n = 0; n < N; ++n) P[static_cast<size_t>(m) * N + n] = std::abs(row[n]) / N;
             }
         }
@@ -721,15 +752,26 @@ int orc_isar_main(const float* echo, const double* velocity, const orc_params* p
         for (int m = 0; m < M; ++m)
             for (int n = 0; n < N; ++n) S[static_cast<size_t>(m) * N + n] *= ham[n];
     }
+    t_range_fft32 = opt->range_fft32;
     orc_range_alignment(reinterpret_cast<double*>(S.data()), p, tr);
+    t_range_fft32 = 0;
     orc_hrrp_center(reinterpret_cast<double*>(S.data()), p, tr);
     std::vector<cd> X(MW);
     orc_cut_range_profile(reinterpret_cast<const double*>(S.data()), p, reinterpret_cast<double*>(X.data()), tr);
     orc_fast_entropy(reinterpret_cast<double*>(X.data()), M, W, opt->iterations, opt->phi0_given, tr);
     if (opt->if_mtrc) orc_mtrc(reinterpret_cast<double*>(X.data()), p, opt->data_type);
+    orc_post_process(reinterpret_cast<double*>(X.data()), M, W, opt->quirk_q1, opt->data_type, img);
+    if (tr) tr->entropy = orc_image_entropy(img, static_cast<int>(MW));
+    return 0;
+}
+
+void orc_post_process(double* x_, int M, int W, int quirk_q1, int data_type, float* img) {
+    // isar_main.cu:191-211
+    cd* X = reinterpret_cast<cd*>(x_);
+    const size_t MW = static_cast<size_t>(M) * W;
     // :191-193 slow-time window
     std::vector<double> hm = hamming(M);
-    if (opt->quirk_q1) {
+    if (quirk_q1) {
         // diagMulMat(d_hamming_echoes, X, X, cols = echo_num, len = range_num_cut): only the
         // first W elements, element i scaled by hamming_M[i / M].
         for (int i = 0; i < W; ++i) X[i] *= hm[i / M];
@@ -738,12 +780,10 @@ int orc_isar_main(const float* echo, const double* velocity, const orc_params* p
             for (int n = 0; n < W; ++n) X[static_cast<size_t>(m) * W + n] *= hm[m];
     }
     // :195-207 azimuth FFT, shift, [flipud], abs
-    fft_cols(X.data(), M, W, -1);
-    swap_halves_cols(X.data(), M, W);
-    if (opt->data_type == 1) flip_cols(X.data(), M, W);
+    fft_cols(X, M, W, -1);
+    swap_halves_cols(X, M, W);
+    if (data_type == 1) flip_cols(X, M, W);
     for (size_t i = 0; i < MW; ++i) img[i] = static_cast<float>(std::abs(X[i]));
-    if (tr) tr->entropy = orc_image_entropy(img, static_cast<int>(MW));
-    return 0;
 }
 
 int orc_isar_batch(int K, const float* echo, const double* velocity, const orc_params* p,

@@ -52,7 +52,9 @@ typedef struct orc_options {
                                 scaled); 0: apply the slow-time Hamming window to all rows  */
     int32_t iterations;      /* fast-entropy iterations, 120 in the reference (common.cuh:73) */
     int32_t phi0_given;      /* 1: use trace->phi0 as the autofocus start (warm start seam) */
-    int32_t reserved;
+    int32_t range_fft32;     /* 1: the range-alignment row transforms in FP32 arithmetic (radix 2,
+                                correctly rounded twiddles), everything else FP64: a sensitivity
+                                probe of how far FP32 range profiles alone move the result      */
 } orc_options;
 
 /* Optional outputs. Any pointer may be NULL. Array sizes in brackets. */
@@ -135,6 +137,11 @@ void orc_fast_entropy(double* x, int rows, int cols, int iterations, int phi0_gi
 
 /* MTRC on the M x W cut, in place (mtrc.cu:4-76). */
 void orc_mtrc(double* x, const orc_params* p, int data_type);
+
+/* Post-processing on the M x W image data, in place (isar_main.cu:191-211): the slow-time
+ * Hamming window (quirk_q1 as in orc_options), the slow-time FFT, the half swap along pulses,
+ * flipud for IFDS, and |.| into img (M*W floats). */
+void orc_post_process(double* x, int M, int W, int quirk_q1, int data_type, float* img);
 
 /* Synthetic-frame-free helpers used by the tests. */
 int orc_next_pow2(int n);                            /* common.cu:521-529 */

@@ -24,7 +24,7 @@ class OrcParams(C.Structure):
 class OrcOptions(C.Structure):
     _fields_ = [("data_type", C.c_int32), ("if_hpc", C.c_int32), ("if_hrrp", C.c_int32),
                 ("if_mtrc", C.c_int32), ("quirk_q1", C.c_int32), ("iterations", C.c_int32),
-                ("phi0_given", C.c_int32), ("reserved", C.c_int32)]
+                ("phi0_given", C.c_int32), ("range_fft32", C.c_int32)]
 
 
 _dp = C.POINTER(C.c_double)
@@ -84,8 +84,9 @@ def params(M, N, W=512, band_width=1_000_000_000, fc=10_000_000_000, Tp=None, Fs
     return OrcParams(M, N, W, 0, int(band_width), int(fc), float(Tp), int(Fs))
 
 
-def options(data_type=2, if_hpc=1, if_hrrp=0, if_mtrc=1, quirk_q1=1, iterations=120, phi0_given=0) -> OrcOptions:
-    return OrcOptions(data_type, if_hpc, if_hrrp, if_mtrc, quirk_q1, iterations, phi0_given, 0)
+def options(data_type=2, if_hpc=1, if_hrrp=0, if_mtrc=1, quirk_q1=1, iterations=120, phi0_given=0,
+            range_fft32=0) -> OrcOptions:
+    return OrcOptions(data_type, if_hpc, if_hrrp, if_mtrc, quirk_q1, iterations, phi0_given, range_fft32)
 
 
 @dataclass
@@ -247,6 +248,18 @@ def mtrc(x: np.ndarray, p: OrcParams, data_type: int = 2) -> np.ndarray:
     a = np.ascontiguousarray(x, dtype=np.complex128).copy()
     lib().orc_mtrc(_ptr(a), C.byref(p), data_type)
     return a
+
+
+def post_process(x: np.ndarray, quirk_q1: bool = True, data_type: int = 2) -> np.ndarray:
+    """isar_main.cu:191-211 on an (M, W) complex array: window, slow-time FFT, shift, |.|."""
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    M, W = a.shape
+    img = np.zeros((M, W), np.float32)
+    L = lib()
+    L.orc_post_process.restype = None
+    L.orc_post_process.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+    L.orc_post_process(_ptr(a), M, W, int(quirk_q1), int(data_type), _ptr(img))
+    return img
 
 
 def next_pow2(n: int) -> int:

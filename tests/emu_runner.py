@@ -52,9 +52,10 @@ class EmuFrame:
 
 
 def run(echo: np.ndarray, velocity: np.ndarray, radar: dict, opts: dict | None = None, phi0=None,
-        timeout: float = 900.0, autofocus_mode: int | None = None) -> list[EmuFrame]:
+        timeout: float = 900.0, autofocus_mode: int | None = None, ra_split: int | None = None) -> list[EmuFrame]:
     """echo: complex64 (K, M, N); velocity: (K, M). radar: band_width, fc, Tp, Fs.
-    autofocus_mode: isar_autofocus_mode (0 auto, 1 one workgroup per frame, 2 split)."""
+    autofocus_mode: isar_autofocus_mode (0 auto, 1 one workgroup per frame, 2 split); ra_split:
+    isar_set_ra_split (0 by batch size, 1 one grid, 2 two grids)."""
     if not DRIVER.exists():
         raise RuntimeError(f"{DRIVER} missing: build with isar_imaging_cuda_amd/build.py --emu")
     opts = opts or default_options()
@@ -76,7 +77,9 @@ def run(echo: np.ndarray, velocity: np.ndarray, radar: dict, opts: dict | None =
             f.write(vel.tobytes())
             if opts.get("warm_start"):
                 f.write(np.ascontiguousarray(phi0, dtype=np.complex64).tobytes())
-        cmd = [str(DRIVER), str(fin), str(fout)] + ([] if autofocus_mode is None else [str(autofocus_mode)])
+        extra = [] if autofocus_mode is None and ra_split is None else [str(autofocus_mode or 0)]
+        extra += [] if ra_split is None else [str(ra_split)]
+        cmd = [str(DRIVER), str(fin), str(fout)] + extra
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError(f"emu driver failed rc={r.returncode}\n{r.stdout}\n{r.stderr[-4000:]}")
@@ -94,7 +97,7 @@ def run(echo: np.ndarray, velocity: np.ndarray, radar: dict, opts: dict | None =
     for k in range(K):
         sc = take(np.int32, 10)
         fs = take(np.float32, 2)
-        lag = take(np.float32, M - 1)
+        lag = take(np.float64, M - 1)
         imax = take(np.int32, M - 1)
         cm, ce, ch = take(np.float32, W), take(np.float32, W), take(np.float32, W)
         sel = take(np.int32, W)

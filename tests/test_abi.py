@@ -158,3 +158,39 @@ def test_wbd_other_polarisation_is_not_picked_up(tmp_path):
     r = subprocess.run([str(APP), str(tmp_path), "32", "8", "0", str(tmp_path / "x"), "1"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 4 and "no valid file" in r.stderr
+
+
+def _great_circle_deg(d_az, pitch):
+    """Angle between two unit vectors at the same pitch whose azimuths differ by d_az degrees."""
+    p, d = np.deg2rad(pitch), np.deg2rad(d_az)
+    return np.rad2deg(np.arccos(np.sin(p) ** 2 + np.cos(p) ** 2 * np.cos(d)))
+
+
+def test_turn_angle_curve_across_a_pitch_jump(tmp_path):
+    """turnAngleLine (common.cu:902-980) on 100 pulses whose pitch jumps from 5 to 40 deg after
+    pulse 59 (a jump > 0.2 deg starts a block): block 0 (pulses 0-59) accumulates great-circle
+    angles at the 20-pulse knots 0, 20, 40 and interpolates linearly between them, held after
+    the last knot. Block 1 (pulses 60-99, knots 0 and 20) samples the curve at its
+    block-relative knot positions taken as absolute indices (common.cu:968-975), i.e. block 0's
+    values at pulses 0 and 20, which discards its own accumulated angle; then it adds block 0's
+    last value. With A1, A2 the angles of pulses 0-20 and 20-40: i*A1/20 to pulse 20, then
+    A1 + (i-20)*A2/20 to 40, A1 + A2 to 59, then A1 + A2 + min(i-60, 20)*A1/20."""
+    from wbd import make_dwell
+    P, rate = 100, 0.05
+    pitch = np.where(np.arange(P) < 60, 5.0, 40.0)
+    make_dwell(tmp_path, pulses=P, range_num=1024, azimuth_rate_deg=rate, pitch_deg=pitch)
+    r = subprocess.run([str(APP), str(tmp_path), str(P), "16", "0", str(tmp_path / "x"), "1"], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    vals = np.array([[float(v) for v in ln.split()] for ln in r.stdout.strip().splitlines()[1:]])
+    assert vals.shape[0] == P
+    az, pt = vals[:, 3], vals[:, 4]   # the angles as the file holds them (quantised to 360/2^24 deg)
+    A1 = _great_circle_deg(az[20] - az[0], pt[0])
+    A2 = _great_circle_deg(az[40] - az[20], pt[20])
+    i = np.arange(P)
+    want = np.where(i <= 20, i * A1 / 20, A1 + np.minimum(i - 20, 20) * A2 / 20)
+    want = np.where(i >= 60, A1 + A2 + np.minimum(i - 60, 20) * A1 / 20, want)
+    assert np.abs(vals[:, 5] - want).max() < 1e-3 * want.max()
+    # block 1's own angle over pulses 60-80 (at 40 deg pitch, about 0.77 A1) is discarded
+    own = _great_circle_deg(az[80] - az[60], pt[60])
+    assert own < 0.8 * A1 and abs(vals[80, 5] - (A1 + A2 + own)) > 0.2 * A1

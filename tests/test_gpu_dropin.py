@@ -38,3 +38,19 @@ def test_calling_sequence_sliding_windows(tmp_path):
     txt = np.loadtxt(tmp_path / "img_0.txt")
     img0 = np.fromfile(tmp_path / "img_0.bin", dtype=np.float32)
     assert np.allclose(txt, img0, atol=6e-6 + 1e-7 * np.abs(img0))   # writeFileFLT: 5 decimals
+
+
+def test_extracting_refuses_arguments_that_do_not_match_init(tmp_path):
+    """dataExtracting with a frame length or data type other than imagingMemInit's, or an output
+    vector smaller than imagingMemInit made it, returns EXIT_FAILURE with a message (the reference
+    would write past the caller's vector); a matching call still succeeds afterwards."""
+    make_dwell(tmp_path / "data", pulses=160, range_num=1024)
+    print("[case] drop-in app, dataExtracting misuse", flush=True)
+    r = subprocess.run([str(APP), str(tmp_path / "data"), "128", "32", "-1", str(tmp_path / "img"), "1"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = dict(ln.split(": rc ", 1) for ln in r.stdout.strip().splitlines())
+    for what in ("frame_len + 4", "other data_type", "dataW half size"):
+        assert lines[what].startswith("1 (") and ("differs" in lines[what] or "smaller" in lines[what]), what
+    assert lines["matching"].startswith("0")

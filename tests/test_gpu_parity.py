@@ -51,16 +51,19 @@ CASES = [
     ("c5", 5, {}),                          # 1024 x 16384, 1000 scatterers, 7 km/s
 ]
 
+# Image bar per case. On the 1024 x 16384 frame the FP64 oracle itself moves by 1.1e-4 when only
+# its range-alignment row transforms run in FP32 with correctly rounded twiddles
+# (tests/test_oracle.py::test_fp32_range_profiles_alone_move_the_large_frame_by_about_the_bar):
+# lags a few 1e-6 cell apart become per-pulse phase differences that 120 autofocus iterations
+# amplify. An FP32 pipeline cannot be held to 1e-4 there; the device (radix-16 transforms, lags
+# from FP64 direct sums) measures 3.7e-4, every integer decision identical and the entropy within
+# 2e-7. Every other case keeps 1e-4.
+IMG_BAR = {"c5": 5e-4}
+
 # Batches of copies of a case's frame large enough (frames x pulses x range cells >= 2^25) for
 # the alignment update's two-grid launch; every copy must equal the single frame bit for bit.
 COPIES = {"c1": 256, "c4": 64, "c2": 32, "c5": 2}
 
-# Image bar per case. At 1024 pulses x 16384 cells the FP32 lags stay within the 1e-3-cell bar
-# (2.2e-4 measured) and every integer decision is identical, but those lag differences and the
-# FP32 Doppler start (3.7e-3 rad over 1024 pulses) are amplified by the 120-iteration fixed point
-# (0.11 rad phase difference after it), so the image differs from the FP64 oracle by 3e-3 relative
-# L2 while its entropy agrees to 2e-6. The 1e-4 image bar holds for every other case.
-IMG_BAR = {"c5": 5e-3}
 
 
 @pytest.mark.parametrize("name,cid,over", CASES, ids=[c[0] for c in CASES])
@@ -71,7 +74,7 @@ def test_pipeline_parity(name, cid, over):
     img, (t,) = _run_device(echo, vel, radar, M, N, isar.Options())
     img_o, tr = oracle_run(echo, vel, radar, M, N)
     rep = compare(img, t, img_o, tr, M, img_rel_l2=IMG_BAR.get(name, IMG_REL_L2))
-    print(rep)
+    print(rep, f"max lag error {np.abs(np.asarray(t.ra_lag) - tr.ra_lag[:M - 1]).max():.2e} cell")
     assert rep.ok(), str(rep)
     n = COPIES.get(name)
     if n:
@@ -193,3 +196,21 @@ def test_batch_matches_single_frames():
         img_s, (ts,) = _run_device(echo[k], vel[k], radar, M, N, isar.Options())
         assert np.array_equal(img_b[k], img_s), f"frame {k} differs between batch and single runs"
         assert tb[k].cut_offset == ts.cut_offset and np.array_equal(tb[k].select_bin, ts.select_bin)
+
+
+@pytest.mark.parametrize("cid,over", [(1, {}), (1, dict(echo_num=192)), (2, {})], ids=["c1", "c1_M192", "c2"])
+def test_ra_update_launch_forms_agree(cid, over):
+    """One alignment-update grid per merge level, or the rows of pair 0 apart from the other
+    pairs (isar_set_ra_split), on a single frame: the same lags, phases and image bit for bit."""
+    cfg, echo, vel, radar = _frame(cid, **over)
+    M, N = cfg.echo_num, cfg.range_num
+    print(f"[case] ra launch forms c{cid} M={M} N={N}", flush=True)
+    params = isar.RadarParams(M, N, radar["band_width"], radar["fc"], radar["Tp"], radar["Fs"])
+    out = []
+    for split in (1, 2):
+        with isar.Imager(params, max_frames=1) as im:
+            im.set_ra_split(split)
+            out.append((im.run_host(echo, vel, isar.Options()), im.trace(0)))
+    (a, ta), (b, tb) = out
+    assert np.array_equal(ta.ra_lag, tb.ra_lag) and np.array_equal(ta.phi, tb.phi)
+    assert np.array_equal(a, b)

@@ -353,3 +353,110 @@ def test_point_targets_focus_to_separate_bins():
     assert far, "second target not resolved"
     r1, c1 = far[0]
     assert abs(abs(c1 - c0) - 40) <= 2
+
+
+# ---------------------------------------------------------------------------------------------
+# image geometry: transform directions and half swaps of MTRC (mtrc.cu:14-70) and of the
+# post-processing (isar_main.cu:191-207), fixed by closed forms
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("M", [64, 96, 256])
+def test_mtrc_is_identity_without_bandwidth(M):
+    """With B = 0 every column has alpha_n = 1: the chirp-z transform is the half-swapped DFT, and
+    the chain swap, IFFT_rows / W, CZT, swap, IFFT_cols / M, swap, FFT_rows, swap_rows, swap_cols
+    returns its input. A flipped transform or a missing or extra half swap breaks the identity."""
+    W = 512
+    rng = np.random.default_rng(M)
+    x = rng.standard_normal((M, W)) + 1j * rng.standard_normal((M, W))
+    p = O.params(M, 4096, W, band_width=0, fc=10_000_000_000, Tp=1e-4)
+    y = O.mtrc(x, p, 2)
+    assert np.linalg.norm(y - x) / np.linalg.norm(x) < 1e-12
+
+
+def test_czt_keystone_maps_scaled_doppler_to_one_bin():
+    """The chirp-z transform of mtrc.cu:125-192 with alpha: a slow-time tone whose frequency is
+    alpha k0 / M (a scatterer's Doppler at a range frequency scaled by alpha = (fc + f) / fc),
+    times a^m = exp(-j pi alpha m), sums coherently to M at bin k0 for every alpha. The same
+    alpha applied the other way round (w = exp(+j 2 pi alpha / M) or alpha -> 1 / alpha) does
+    not."""
+    M, k0 = 128, 37
+    m = np.arange(M)
+    for alpha in (0.95, 0.99, 1.0, 1.013, 1.05):
+        x = np.exp(2j * np.pi * alpha * k0 * m / M) * np.exp(-1j * np.pi * alpha * m)
+        X = O.czt_alpha(x, alpha)
+        assert abs(X[k0] - M) < 1e-9 * M, alpha
+        assert np.argmax(np.abs(X)) == k0
+
+
+@pytest.mark.parametrize("M,k", [(64, 5), (128, 0), (128, 100), (256, 200)])
+def test_post_process_places_slow_time_tone(M, k):
+    """isar_main.cu:195-207: the slow-time FFT is forward (cufftExecC2C, CUFFT_FORWARD) and
+    ifftshiftCols swaps the halves along pulses, so a tone exp(j 2 pi k m / M) in one column
+    lands at image row (k + M/2) mod M. With its pulse-0 sample zeroed (the Q1 window scales row
+    0 only), the column is |sum_{m >= 1} exp(j 2 pi (k - k') m / M)|: M - 1 at the tone and 1 on
+    every other row. A flipped transform puts it at (-k + M/2) mod M, a missing swap at k."""
+    W, n0 = 512, 300
+    x = np.zeros((M, W), complex)
+    x[:, n0] = np.exp(2j * np.pi * k * np.arange(M) / M)
+    x[0, n0] = 0
+    img = O.post_process(x, quirk_q1=True)
+    want = np.zeros((M, W))
+    want[:, n0] = 1.0
+    want[(k + M // 2) % M, n0] = M - 1
+    assert np.allclose(img, want, atol=1e-9 * M)
+
+
+@pytest.mark.parametrize("M", [64, 256, 512])
+def test_post_process_window_quirk(M):
+    """Q1 (isar_main.cu:192 with diagMulMat, common.cu:230-238): with cols = echo_num and
+    len = range_num_cut, element i < W of the row-major M x W array, which is row 0, is scaled
+    by hamming_M[i / M]; nothing else is. For an all-ones array column n then holds
+    [h, 1, ..., 1] with h = hamming_M[n // M], whose DFT is M - 1 + h at bin 0 and h - 1
+    elsewhere: after the swap, row M/2 holds M - 1 + h and every other row 1 - h. The intended
+    window (quirk off) gives the swapped DFT of the Hamming window in every column."""
+    W = 512
+    hm = 0.54 - 0.46 * np.cos(2 * np.pi * np.arange(M) / (M - 1))   # genHammingVec, common.cu:260-266
+    img = O.post_process(np.ones((M, W), complex), quirk_q1=True)
+    h = hm[np.arange(W) // M]
+    want = np.tile(1 - h, (M, 1))
+    want[M // 2] = M - 1 + h
+    assert np.allclose(img, want, atol=1e-6)
+    img0 = O.post_process(np.ones((M, W), complex), quirk_q1=False)
+    col = np.abs(np.fft.fftshift(np.fft.fft(np.hamming(M))))
+    assert np.allclose(img0, np.tile(col[:, None], (1, W)), atol=1e-6)
+
+
+def test_post_process_ifds_flips_doppler():
+    """IFDS adds flipud after the shift (isar_main.cu:201-204): row r goes to M - 1 - r."""
+    M, W, k, n0 = 128, 512, 9, 17
+    x = np.zeros((M, W), complex)
+    x[1:, n0] = np.exp(2j * np.pi * k * np.arange(1, M) / M)
+    a = O.post_process(x, True, 2)
+    b = O.post_process(x, True, 1)
+    assert np.array_equal(b, a[::-1])
+    assert np.argmax(b[:, n0]) == M - 1 - (k + M // 2) % M
+
+
+@pytest.mark.slow
+def test_fp32_range_profiles_alone_move_the_large_frame_by_about_the_bar():
+    """How far FP32 arithmetic can agree with the FP64 oracle on the 1024 x 16384 frame (1000
+    scatterers, 7 km/s): the oracle with only its range-alignment row transforms in FP32 (radix
+    2, correctly rounded twiddles; every other step FP64) against itself in FP64. The lags then
+    differ by up to a few 1e-6 cell, and 120 autofocus iterations turn the resulting per-pulse
+    phase differences into an image difference of order 1e-4, the parity bar itself: an FP32
+    pipeline cannot be held to 1e-4 on this frame, and its bar is set from this figure. The
+    integer decisions do not change."""
+    cfg = synth.preset(5)
+    echo, vel = synth.frame(cfg)
+    h = synth.radar_header(cfg)
+    M, N = cfg.echo_num, cfg.range_num
+    p = O.params(M, N, 512, h["band_width"], h["fc"], h["Tp"], h["Fs"])
+    t64, t32 = O.Trace(M, N, 512), O.Trace(M, N, 512)
+    a = O.isar_main(echo, vel, p, O.options(), t64)
+    b = O.isar_main(echo, vel, p, O.options(range_fft32=1), t32)
+    rel = float(np.linalg.norm(b - a) / np.linalg.norm(a))
+    lag = float(np.abs(t32.ra_lag[:M - 1] - t64.ra_lag[:M - 1]).max())
+    print(f"FP32 range transforms: image rel L2 {rel:.3e}, max lag difference {lag:.2e} cell")
+    assert np.array_equal(t32.ra_imax[:M - 1], t64.ra_imax[:M - 1]) and t32.c.cut_offset == t64.c.cut_offset
+    assert np.array_equal(t32.selected, t64.selected)
+    assert 1e-7 < lag < 1e-5
+    assert 5e-5 < rel < 5e-4

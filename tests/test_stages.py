@@ -23,8 +23,8 @@ APP = ROOT / "isar_imaging_cuda_amd" / "lib" / "isar_calling_app"
 # Bars per stage (relative L2 against the oracle stage fed with the device's previous stage).
 HPC_REL = 1e-6       # FP32 samples times an FP64-reduced phase
 HRRP_REL = 1e-5      # one FP32 transform of <= 16384 points
-RA_REL = {"c1": 1e-4, "c2": 1e-4, "c5": 2e-3}   # fractional lags in FP32 (<= 2.2e-4 cell at c5)
-PC_REL = 1e-4        # cut, Doppler tracking and 120 autofocus iterations (c1, c2)
+RA_REL = 1e-4        # FP32 row transforms, FP64 lags
+PC_REL = 1e-4        # cut, Doppler tracking and 120 autofocus iterations
 MTRC_REL = 1e-5      # chirp-z transform with FP64 chirps, FP32 transforms
 
 
@@ -59,15 +59,14 @@ def test_stage_by_stage(name, cid):
     ra_o = O.hrrp_center(O.range_alignment(st["hpc"], p, tr), p, tr)
     r["ra"] = _rel(st["ra"], ra_o)
     r["mtrc"] = _rel(st["mtrc"], O.mtrc(st["pc"], p, 2))
-    if name != "c5":   # at 1024 pulses the autofocus fixed point amplifies FP32 differences
-        tr2 = O.Trace(M, N, 512)
-        r["pc"] = _rel(st["pc"], O.fast_entropy(O.cut_range_profile(st["ra"], p, tr2), 120, tr2))
+    tr2 = O.Trace(M, N, 512)
+    r["pc"] = _rel(st["pc"], O.fast_entropy(O.cut_range_profile(st["ra"], p, tr2), 120, tr2))
     print({k: f"{v:.2e}" for k, v in r.items()}, f"shift dev {t.center_shift} oracle {tr.center_shift}")
     assert t.center_shift == tr.center_shift
     assert r["hpc"] <= HPC_REL and r["hrrp"] <= HRRP_REL
-    assert r["ra"] <= RA_REL[name]
+    assert r["ra"] <= RA_REL
     assert r["mtrc"] <= MTRC_REL
-    assert r.get("pc", 0.0) <= PC_REL
+    assert r["pc"] <= PC_REL
     assert np.array_equal(st["final"], img)
 
 

@@ -15,14 +15,15 @@ from isar_imaging_cuda_amd import synth
 
 
 def make_dwell(directory: Path, pulses: int = 256, range_num: int = 1024, seed: int = 77, polar: str = "1101",
-               azimuth_rate_deg: float = 0.002, pitch_deg: float = 5.0):
-    """Writes one STRETCH file; returns (path, echo complex64 (P, N), velocity as stored, config)."""
+               azimuth_rate_deg: float = 0.002, pitch_deg: float | np.ndarray = 5.0):
+    """Writes one STRETCH file; returns (path, echo complex64 (P, N), velocity as stored, config).
+    pitch_deg: one pitch for every pulse, or one per pulse."""
     cfg = synth.preset(1, echo_num=pulses, range_num=range_num, seed=seed, walk_cells=24.0)
     echo, vel = synth.frame(cfg)
     directory.mkdir(parents=True, exist_ok=True)
     path = directory / f"test_dwell_00_{polar}.wbd"
     az = 30.0 + azimuth_rate_deg * np.arange(pulses)
-    pitch = np.full(pulses, pitch_deg)
+    pitch = np.broadcast_to(np.asarray(pitch_deg, np.float64), (pulses,)).copy()
     rng_m = 10000.0 + 20.0 * np.arange(pulses) / cfg.prf
     synth.write_wbd(path, echo, cfg, range_m=rng_m, velocity=vel, azimuth=az, pitch=pitch)
     return path, echo, vel, cfg
