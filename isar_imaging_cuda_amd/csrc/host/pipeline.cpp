@@ -72,25 +72,61 @@ int validate_params(const isar_radar_params& p) {
     return ISAR_OK;
 }
 
-size_t workspace_bytes(const isar_radar_params& p, int K) {
-    const size_t M = p.echo_num, N = p.range_num, W = p.range_num_cut;
+// Places the per-batch buffers of K frames and the shared tables in one region starting at base
+// (256-byte aligned), each buffer at a 256-byte boundary and sized K times its per-frame bytes
+// rounded up to 256, so the total grows linearly with K. With base == nullptr it only counts.
+// Returns the bytes the region needs; fills J's workspace pointers and *tables when base is set.
+size_t carve_workspace(const isar_radar_params& p, int K_, char* base, Job* J, SharedTables* tables) {
+    const size_t K = K_, M = p.echo_num, N = p.range_num, W = p.range_num_cut;
     const size_t F = next_pow2_ref(2 * p.echo_num - 1);
-    const size_t nchunks = M / 2;
+    const size_t nchunks = M / 2;   // final-row chunks are indexed by head row / 2
     const size_t nbands = (M + kCutBandRows - 1) / kCutBandRows;
-    size_t per = 0;
-    per += M * N * sizeof(float2);              // pre / st
-    per += 2 * M * N * sizeof(float);           // egrp, epart
-    per += nchunks * N * sizeof(float) + nchunks * sizeof(float);
-    per += 2 * M * sizeof(double) + M * sizeof(float);          // lagsum, lags, imax
-    per += sizeof(FrameDecisions);
-    per += 4 * M * W * sizeof(float2);          // x, stm, czt, su
-    per += M * sizeof(float2) * 3;              // rowdop, phi0, phi
-    per += nbands * W * sizeof(float) + 3 * W * sizeof(float) + W * sizeof(int32_t) + W * sizeof(float2);
-    per += W * sizeof(float2);                  // delta
-    per += 2 * (W / 8) * M * sizeof(float2);    // af_part (two iterations' halves)
-    // shared: chirp-z filter spectra, input/output chirps, range window
-    return per * K + W * F * sizeof(float2) + 2 * W * M * sizeof(float2) + N * sizeof(float);
+    auto r256 = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    size_t off = 0;
+    auto take = [&](size_t per_frame, size_t count) -> void* {
+        void* ptr = base ? base + off : nullptr;
+        off += count * r256(per_frame ? per_frame : 16);
+        return ptr;
+    };
+    auto frame = [&](size_t bytes) { return take(bytes, K); };
+    auto shared = [&](size_t bytes) { return take(bytes, 1); };
+    Job tmp{};
+    Job& j = J ? *J : tmp;
+    j.pre = static_cast<float2*>(frame(M * N * sizeof(float2)));
+    j.st = j.pre;   // each row is read completely before its aligned profile overwrites it
+    j.egrp = static_cast<float*>(frame(M * N * sizeof(float)));
+    j.epart = static_cast<float*>(frame(M * N * sizeof(float)));
+    j.arp_part = static_cast<float*>(frame(nchunks * N * sizeof(float)));
+    j.max_part = static_cast<float*>(frame(nchunks * sizeof(float)));
+    j.lagsum = static_cast<double*>(frame(M * sizeof(double)));
+    j.lags = static_cast<double*>(frame(M * sizeof(double)));
+    j.imax = static_cast<int32_t*>(frame(M * sizeof(int32_t)));
+    j.dec = static_cast<FrameDecisions*>(frame(sizeof(FrameDecisions)));
+    j.x = static_cast<float2*>(frame(M * W * sizeof(float2)));
+    j.rowdop = static_cast<float2*>(frame(M * sizeof(float2)));
+    j.colmax_part = static_cast<float*>(frame(nbands * W * sizeof(float)));
+    j.colmax = static_cast<float*>(frame(W * sizeof(float)));
+    j.colE = static_cast<float*>(frame(W * sizeof(float)));
+    j.colH = static_cast<float*>(frame(W * sizeof(float)));
+    j.sel = static_cast<int32_t*>(frame(W * sizeof(int32_t)));
+    j.su = static_cast<float2*>(frame(W * M * sizeof(float2)));
+    j.phi0 = static_cast<float2*>(frame(M * sizeof(float2)));
+    j.phi = static_cast<float2*>(frame(M * sizeof(float2)));
+    j.stm = static_cast<float2*>(frame(M * W * sizeof(float2)));
+    j.czt = static_cast<float2*>(frame(M * W * sizeof(float2)));
+    j.colsum = static_cast<float2*>(frame(W * sizeof(float2)));
+    j.delta = static_cast<float2*>(frame(W * sizeof(float2)));
+    j.af_part = static_cast<float2*>(frame(2 * (W / 8) * M * sizeof(float2)));
+    SharedTables t{};
+    t.czt_table = static_cast<float2*>(shared(W * F * sizeof(float2)));
+    t.czt_pre = static_cast<float2*>(shared(W * M * sizeof(float2)));
+    t.czt_post = static_cast<float2*>(shared(W * M * sizeof(float2)));
+    t.win_range = static_cast<float*>(shared(N * sizeof(float)));
+    if (tables) *tables = t;
+    return off;
 }
+
+size_t workspace_bytes(const isar_radar_params& p, int K) { return carve_workspace(p, K, nullptr, nullptr, nullptr); }
 
 Context::~Context() {
     drop_graph();
@@ -139,79 +175,51 @@ int Context::init(const isar_radar_params& params, int frames) {
     int rc = validate_params(params);
     if (rc) return rc;
     if (frames < 1) return fail(ISAR_ERR_INVALID, "max_frames must be >= 1");
+    const size_t bytes = workspace_bytes(params, frames);
+    void* base = nullptr;
+    if (hipMalloc(&base, bytes) != hipSuccess)
+        return fail(ISAR_ERR_MEMORY, "device allocation failed for " + std::to_string(frames) + " frames");
+    allocations.push_back(base);
+    rc = init_in(params, frames, base, nullptr);
+    if (rc) return rc;
+    const hipError_t e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("workspace tables: ") + hipGetErrorString(e));
+    return ISAR_OK;
+}
+
+int Context::init_in(const isar_radar_params& params, int frames, void* base, hipStream_t s) {
+    int rc = validate_params(params);
+    if (rc) return rc;
+    if (frames < 1) return fail(ISAR_ERR_INVALID, "max_frames must be >= 1");
     hipError_t ce = configure_kernels();
     if (ce != hipSuccess)
         return fail(ISAR_ERR_DEVICE, std::string("kernel configuration: ") + hipGetErrorString(ce) + " " + g_config_error);
     p = params;
     max_frames = frames;
-    const size_t K = frames, M = p.echo_num, N = p.range_num, W = p.range_num_cut;
-    F = next_pow2_ref(2 * p.echo_num - 1);
-    const size_t nchunks = M / 2;   // final-row chunks are indexed by head row / 2
-    const size_t nbands = (M + kCutBandRows - 1) / kCutBandRows;
-    bool ok = true;
-    auto alloc = [&](size_t bytes) -> void* {
-        void* ptr = nullptr;
-        if (!ok) return nullptr;
-        if (hipMalloc(&ptr, bytes ? bytes : 16) != hipSuccess) { ok = false; return nullptr; }
-        allocations.push_back(ptr);
-        return ptr;
-    };
     Job& J = tmpl;
-    J.M = static_cast<int>(M);
-    J.N = static_cast<int>(N);
-    J.W = static_cast<int>(W);
+    J.M = p.echo_num;
+    J.N = p.range_num;
+    J.W = p.range_num_cut;
+    F = next_pow2_ref(2 * p.echo_num - 1);
     J.F = F;
-    J.pre = static_cast<float2*>(alloc(K * M * N * sizeof(float2)));
-    J.st = J.pre;   // each row is read completely before its aligned profile overwrites it
-    J.egrp = static_cast<float*>(alloc(K * M * N * sizeof(float)));
-    J.epart = static_cast<float*>(alloc(K * M * N * sizeof(float)));
-    J.arp_part = static_cast<float*>(alloc(K * nchunks * N * sizeof(float)));
-    J.max_part = static_cast<float*>(alloc(K * nchunks * sizeof(float)));
-    J.lagsum = static_cast<double*>(alloc(K * M * sizeof(double)));
-    J.lags = static_cast<double*>(alloc(K * M * sizeof(double)));
-    J.imax = static_cast<int32_t*>(alloc(K * M * sizeof(int32_t)));
-    J.dec = static_cast<FrameDecisions*>(alloc(K * sizeof(FrameDecisions)));
-    J.x = static_cast<float2*>(alloc(K * M * W * sizeof(float2)));
-    J.rowdop = static_cast<float2*>(alloc(K * M * sizeof(float2)));
-    J.colmax_part = static_cast<float*>(alloc(K * nbands * W * sizeof(float)));
-    J.colmax = static_cast<float*>(alloc(K * W * sizeof(float)));
-    J.colE = static_cast<float*>(alloc(K * W * sizeof(float)));
-    J.colH = static_cast<float*>(alloc(K * W * sizeof(float)));
-    J.sel = static_cast<int32_t*>(alloc(K * W * sizeof(int32_t)));
-    J.su = static_cast<float2*>(alloc(K * W * M * sizeof(float2)));
-    J.phi0 = static_cast<float2*>(alloc(K * M * sizeof(float2)));
-    J.phi = static_cast<float2*>(alloc(K * M * sizeof(float2)));
-    J.stm = static_cast<float2*>(alloc(K * M * W * sizeof(float2)));
-    J.czt = static_cast<float2*>(alloc(K * M * W * sizeof(float2)));
-    J.colsum = static_cast<float2*>(alloc(K * W * sizeof(float2)));
-    czt_table = static_cast<float2*>(alloc(W * F * sizeof(float2)));
-    float2* czt_pre = static_cast<float2*>(alloc(W * M * sizeof(float2)));
-    float2* czt_post = static_cast<float2*>(alloc(W * M * sizeof(float2)));
-    J.delta = static_cast<float2*>(alloc(K * W * sizeof(float2)));
-    J.af_part = static_cast<float2*>(alloc(2 * K * (W / 8) * M * sizeof(float2)));
-    float* win_range = static_cast<float*>(alloc(N * sizeof(float)));
-    if (!ok) return fail(ISAR_ERR_MEMORY, "device allocation failed for " + std::to_string(frames) + " frames");
+    SharedTables t{};
+    carve_workspace(p, frames, static_cast<char*>(base), &J, &t);
     // constants (high_speed_compensation.cu:16-18, mtrc.cu:10-12, 47)
     const double B = static_cast<double>(p.band_width), fc = static_cast<double>(p.fc), Fs = static_cast<double>(p.Fs);
     J.hpc_k = -4.0 * (B / p.Tp) / kLightSpeed / (Fs * Fs);
     J.czt_c0 = B / (2.0 * fc);
-    J.czt_posa = B / (fc * (static_cast<double>(W) - 1.0));
-    J.czt_table = czt_table;
-    {   // Hamming window over the range cells, evaluated in double on the host once
-        std::vector<float> w(N);
-        for (size_t n = 0; n < N; ++n)
-            w[n] = static_cast<float>(0.54 - 0.46 * std::cos(2.0 * kPi * static_cast<double>(n) / static_cast<double>(N - 1)));
-        if (hipMemcpy(win_range, w.data(), N * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
-            return fail(ISAR_ERR_DEVICE, "window upload failed");
-        J.win_range = win_range;
-    }
-    launch_czt_table(J.M, J.W, F, J.czt_c0, J.czt_posa, czt_table, nullptr);
-    launch_czt_chirps(J.M, J.W, F, J.czt_c0, J.czt_posa, czt_pre, czt_post, nullptr);
-    J.czt_pre = czt_pre;
-    J.czt_post = czt_post;
-    hipError_t e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipGetLastError();
-    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("chirp-z table: ") + hipGetErrorString(e));
+    J.czt_posa = B / (fc * (static_cast<double>(J.W) - 1.0));
+    czt_table = t.czt_table;
+    J.czt_table = t.czt_table;
+    J.czt_pre = t.czt_pre;
+    J.czt_post = t.czt_post;
+    J.win_range = t.win_range;
+    // the range Hamming window and the chirp-z tables, formed on the device on stream s
+    launch_range_window(J.N, t.win_range, s);
+    launch_czt_table(J.M, J.W, F, J.czt_c0, J.czt_posa, t.czt_table, s);
+    launch_czt_chirps(J.M, J.W, F, J.czt_c0, J.czt_posa, t.czt_pre, t.czt_post, s);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("workspace tables: ") + hipGetErrorString(e));
     return ISAR_OK;
 }
 

@@ -46,6 +46,9 @@ struct Context {
 
     ~Context();
     int init(const isar_radar_params& params, int frames);
+    // the same over a caller's workspace of workspace_bytes(params, frames) bytes at base (256-byte
+    // aligned): no allocation; the tables are formed by kernels on stream s, asynchronously
+    int init_in(const isar_radar_params& params, int frames, void* base, hipStream_t s);
     int run(int frames, const void* echo, const double* velocity, const void* phi0, float* img,
             const isar_options& opt, hipStream_t stream);
     // frame f reads echo rows frame_row[f] .. + M - 1 and writes its image at img + frame_img[f]
@@ -97,6 +100,14 @@ struct Context {
 
 int next_pow2_ref(int n);   // common.cu:521-529
 size_t workspace_bytes(const isar_radar_params& p, int frames);
+// tables shared by every frame of a context: chirp-z filter spectra and chirps, range window
+struct SharedTables {
+    float2* czt_table;
+    float2* czt_pre;
+    float2* czt_post;
+    float* win_range;
+};
+size_t carve_workspace(const isar_radar_params& p, int frames, char* base, Job* J, SharedTables* tables);
 int validate_params(const isar_radar_params& p);
 
 }  // namespace isar

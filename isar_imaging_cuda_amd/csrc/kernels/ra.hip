@@ -488,6 +488,13 @@ __global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) voi
     for (int e = 0; e < kE; ++e) dst[out_pos<N>(tid, e)] = acc[e];
 }
 
+// Range Hamming window 0.54 - 0.46 cos(2 pi n / (N - 1)) (genHammingVec, common.cu:260-266),
+// evaluated in FP64 and rounded once to FP32.
+__global__ __launch_bounds__(256) void k_range_window(int N, float* win) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n < N) win[n] = static_cast<float>(0.54 - 0.46 * cos(2.0 * 3.14159265358979323846 * n / (N - 1)));
+}
+
 // ------------------------------------------------------------------------------------------
 // host launchers
 // ------------------------------------------------------------------------------------------
@@ -532,6 +539,10 @@ hipError_t configure_ra_kernels() {
 #undef ISAR_SET_RA
 #endif
     return err;
+}
+
+void launch_range_window(int N, float* win, hipStream_t s) {
+    ISAR_LAUNCH(k_range_window, dim3((N + 255) / 256), dim3(256), 0, s, N, win);
 }
 
 void launch_ifds_pre(const Job& J, hipStream_t s) {

@@ -33,6 +33,7 @@ void launch_autofocus_split(const Job& J, bool fold, hipStream_t s);
 
 // MTRC and azimuth compression (mtrc.cu:4-192, isar_main.cu:191-207)
 void launch_czt_table(int M, int W, int F, double c0, double posa, float2* table, hipStream_t s);
+void launch_range_window(int N, float* win, hipStream_t s);
 void launch_czt_chirps(int M, int W, int F, double c0, double posa, float2* pre, float2* post, hipStream_t s);
 void launch_mtrc_delta(const Job& J, hipStream_t s);
 void launch_mtrc_rows(const Job& J, hipStream_t s);
