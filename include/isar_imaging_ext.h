@@ -102,6 +102,20 @@ ISAR_API size_t isar_workspace_bytes(const isar_radar_params* p, int max_frames)
 ISAR_API int isar_run_device(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity,
                              const void* d_phi0, float* d_img, const isar_options* opt, void* stream);
 
+/* isar_run_device without a context and without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under stream capture,
+ * tight serving loops, hosts that forbid hipMalloc). The workspace is a device array the library
+ * holds per GPU until the process ends, isar_static_workspace_bytes() bytes (64 MiB: two frames
+ * of 256 x 4096, one of 512 x 4096); a batch needing more (isar_workspace_bytes) is refused
+ * with ISAR_ERR_INVALID. The chirp-z and window tables are formed on `stream` at every call.
+ * Full runs only (stage_mode 0). Every call on one GPU shares that GPU's array: calls whose
+ * work overlaps on one GPU race. The caller passes device inputs and outputs as for
+ * isar_run_device. */
+ISAR_API int isar_run_device_static(const isar_radar_params* p, int frames, const void* d_echo,
+                                    const double* d_velocity, const void* d_phi0, float* d_img,
+                                    const isar_options* opt, void* stream);
+ISAR_API size_t isar_static_workspace_bytes(void);
+
 /* Device-resident batch whose frames are windows of a longer recording: frame f uses echo rows
  * (and velocities) d_frame_rows[f] .. d_frame_rows[f] + M - 1 of d_echo (N complex floats per
  * row) and writes its M x W image at d_img + d_frame_img[f] floats. Both tables are device

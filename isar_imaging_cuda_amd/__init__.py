@@ -340,6 +340,25 @@ class Imager:
                           phi=p1[0::2] + 1j * p1[1::2])
 
 
+def static_workspace_bytes() -> int:
+    """Size of the per-GPU device array isar_run_device_static images in."""
+    L = lib()
+    L.isar_static_workspace_bytes.restype = C.c_size_t
+    return int(L.isar_static_workspace_bytes())
+
+
+def run_device_static(params: RadarParams, echo_ptr: int, velocity_ptr: int | None, img_ptr: int, frames: int,
+                      options: Options | None = None, phi0_ptr: int | None = None, stream: int | None = None) -> None:
+    """isar_run_device without a context or any device allocation: the workspace is the library's
+    per-GPU static array (static_workspace_bytes()); calls overlapping on one GPU race."""
+    options = options or Options()
+    L = lib()
+    vp = C.c_void_p
+    L.isar_run_device_static.argtypes = [C.POINTER(_Params), C.c_int, vp, vp, vp, vp, C.POINTER(_Options), vp]
+    _check(L.isar_run_device_static(C.byref(params.c()), frames, vp(echo_ptr), vp(velocity_ptr or 0), vp(phi0_ptr or 0),
+                                    vp(img_ptr), C.byref(options.c()), vp(stream or 0)), "isar_run_device_static")
+
+
 def fft(x: np.ndarray, inverse: bool = False) -> np.ndarray:
     """Unnormalised device FFT of each row of a 2-D complex array (tests and diagnostics)."""
     a = np.ascontiguousarray(x, dtype=np.complex64)

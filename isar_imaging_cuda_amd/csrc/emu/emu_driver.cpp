@@ -1,11 +1,14 @@
 // emu_driver.cpp — runs the kernel sources on the CPU (hip_emu.hpp) through the same C ABI as
 // the device library. Test infrastructure: built with AddressSanitizer by the CPU test suite.
 //
-// usage: isar_emu_driver <input.bin> <output.bin> [autofocus_mode [ra_split]]
+// usage: isar_emu_driver <input.bin> <output.bin> [autofocus_mode [ra_split [static]]]
+// static = 1: also images the batch, then its first frame alone, through isar_run_device_static
+// and requires both to equal the context's images bit for bit (exit 10-12 otherwise).
 // input : int32 M, N, W, K; int64 band_width, fc; double Tp; int64 Fs; isar_options;
 //         echo K*M*N complex float; velocity K*M double; [phi0 K*M complex float if warm_start]
 // output: img K*M*W float; then per frame: int32[10] scalars, float[2], double lag[M-1], imax[M-1],
 //         colmax[W], colE[W], colH[W], select_bin[W], phi0[2M], phi[2M]
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -13,8 +16,8 @@
 #include "isar_imaging_ext.h"
 
 int main(int argc, char** argv) {
-    if (argc < 3 || argc > 5) {
-        std::fprintf(stderr, "usage: %s input.bin output.bin [autofocus_mode [ra_split]]\n", argv[0]);
+    if (argc < 3 || argc > 6) {
+        std::fprintf(stderr, "usage: %s input.bin output.bin [autofocus_mode [ra_split [static]]]\n", argv[0]);
         return 2;
     }
     FILE* in = std::fopen(argv[1], "rb");
@@ -52,6 +55,20 @@ int main(int argc, char** argv) {
     if (isar_run_host(ctx, K, echo.data(), vel.data(), opt.warm_start ? phi0.data() : nullptr, img.data(), &opt) != 0) {
         std::fprintf(stderr, "run: %s\n", isar_last_error());
         return 7;
+    }
+    if (argc == 6 && std::atoi(argv[5]) == 1) {
+        // the emulated device memory is host memory: the host arrays serve as device buffers
+        std::vector<float> img_s(img.size());
+        const void* ph = opt.warm_start ? phi0.data() : nullptr;
+        if (isar_run_device_static(&p, K, echo.data(), vel.data(), ph, img_s.data(), &opt, nullptr) != 0) {
+            std::fprintf(stderr, "static run: %s\n", isar_last_error());
+            return 10;
+        }
+        if (img_s != img) return 11;
+        std::vector<float> one(static_cast<size_t>(M) * W);
+        if (isar_run_device_static(&p, 1, echo.data(), vel.data(), ph, one.data(), &opt, nullptr) != 0 ||
+            !std::equal(one.begin(), one.end(), img.begin()))
+            return 12;
     }
     FILE* out = std::fopen(argv[2], "wb");
     if (!out) return 8;

@@ -2,7 +2,10 @@
 UndefinedBehaviorSanitizer on) against the FP64 oracle: indexing, LDS use and barrier structure
 checked without a GPU, in both launch forms of the range-alignment update (one grid per merge
 level, and the rows of pair 0 apart from the other pairs, isar_set_ra_split). A 192-pulse frame
-carries an odd group up a level (range_alignment.cu:34-73 generalised to any even M)."""
+carries an odd group up a level (range_alignment.cu:34-73 generalised to any even M). Emulated
+device memory starts as all-ones bytes (NaN floats), so a kernel reading workspace it did not
+write in the same run would show; the static-workspace entry point (isar_run_device_static) must
+equal the context run bit for bit, on a first call and on a second, smaller one."""
 from __future__ import annotations
 
 import concurrent.futures as cf
@@ -20,6 +23,7 @@ pytestmark = pytest.mark.slow
 # a few autofocus iterations keep the emulated run to about a minute; the oracle runs as many
 ITERATIONS = 4
 CASES = [(128, 1, "c1, one grid"), (128, 2, "c1, two grids"), (192, 2, "M=192, two grids")]
+STATIC_CASE = (64, 2, "M=64 x 2 frames, static workspace")
 
 
 @pytest.fixture(scope="module")
@@ -37,10 +41,31 @@ def _one(M, split):
     return fr, compare(fr.img, fr, img_o, tr, M), tr
 
 
+def _static(M, frames):
+    """Two frames through the context, then through isar_run_device_static (both frames, then the
+    first alone); the driver exits non-zero unless all three agree bit for bit."""
+    cfgs = [synth.preset(1, echo_num=M, range_num=1024, seed=40 + k) for k in range(frames)]
+    fr = [synth.frame(c) for c in cfgs]
+    radar = synth.radar_header(cfgs[0])
+    echo, vel = np.stack([f[0] for f in fr]), np.stack([f[1] for f in fr])
+    out = emu_runner.run(echo, vel, radar, emu_runner.default_options(iterations=ITERATIONS), static_check=True,
+                         timeout=600)
+    reps = []
+    for k in range(frames):
+        img_o, tr = oracle_run(echo[k], vel[k], radar, M, 1024, iterations=ITERATIONS)
+        reps.append(compare(out[k].img, out[k], img_o, tr, M))
+    return reps
+
+
 def test_emulated_kernels_match_oracle(emu_driver):
-    with cf.ThreadPoolExecutor(max_workers=len(CASES)) as ex:
+    with cf.ThreadPoolExecutor(max_workers=len(CASES) + 1) as ex:
         futs = {name: ex.submit(_one, M, split) for M, split, name in CASES}
+        fstatic = ex.submit(_static, STATIC_CASE[0], STATIC_CASE[1])
         res = {name: f.result() for name, f in futs.items()}
+        static_reps = fstatic.result()
+    for rep in static_reps:
+        print(STATIC_CASE[2], rep)
+        assert rep.ok(), f"{STATIC_CASE[2]}: {rep}"
     for name, (fr, rep, tr) in res.items():
         print(name, rep)
         assert rep.ok(), f"{name}: {rep}"
