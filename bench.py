@@ -124,6 +124,16 @@ def learned_vs_plain(frames: int = 1000) -> dict:
                      front_iters=front_iters)
     res["front_iterations"] = front_iters
     res["workload"] = "256 x 2048 complex64 frames, held-out seeds"
+    # device time per frame of the autofocus and what follows it, at the mean iteration counts:
+    # plain = its iterations + the image; learned = its iterations (the front ones included) +
+    # the network + the judgement of the start (three image formations) + the image
+    t = L.stage_times(params, list(range(30_000_001, 30_000_257)), front_iters=front_iters)
+    plain_ms = res["plain_mean_iterations"] * t["iteration"] + t["image"]
+    learned_ms = res["learned_mean_iterations"] * t["iteration"] + t["network"] + t["judge"] + t["image"]
+    res["device_ms_per_frame"] = {"iteration": round(t["iteration"], 6), "image": round(t["image"], 6),
+                                  "network": round(t["network"], 6), "judge": round(t["judge"], 6),
+                                  "plain_chain": round(plain_ms, 6), "learned_chain": round(learned_ms, 6),
+                                  "time_ratio": round(plain_ms / learned_ms, 3), "batch_frames": t["batch_frames"]}
     return res
 
 
@@ -158,10 +168,17 @@ def subapertures(count: int = 64, replays: int = 10) -> dict:
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / replays
     plan.close()
+    # minimum-pass bytes of every window of the launch; U = 64 selected bins per window (the 8MU
+    # term is at most 0.13 % of a window's bytes at 512 x 4096, so the assumption barely matters)
+    alg = float(sum(algorithmic_bytes_per_frame(int(m), N, W, 64.0) for m in pulses))
+    gbs = alg / (ms / 1e3) / 1e9
     return {"workload": f"{count} windows of one {P} x {N} dwell, pulse counts {sorted(set(pulses.tolist()))}",
             "ms_per_graph_launch": round(ms, 4), "windows_per_s": round(count / (ms / 1e3), 1),
             "pulses_per_s": round(float(pulses.sum()) / (ms / 1e3), 1),
-            "ra_chunk_rows": {int(m): int(c) for m, c in zip(plan.group_pulses, chosen)}}
+            "ra_chunk_rows": {int(m): int(c) for m, c in zip(plan.group_pulses, chosen)},
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                         "algorithmic_bytes_per_launch": alg, "selected_bins_assumed": 64}}
 
 
 def large_batch(frames: int = 8, distinct: int = 2, steps: int = 3) -> dict:

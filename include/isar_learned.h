@@ -68,6 +68,14 @@ ISAR_API int isar_image_entropy(int frames, int n, const float* d_img, double* d
  * (the first on ties). */
 ISAR_API int isar_select_phase(int frames, int M, const void* d_phi_a, const double* d_ent_a, const void* d_phi_b,
                                const double* d_ent_b, void* d_out, double* d_ent_out, void* stream);
+/* The final judgement (the plain autofocus as fallback and judge, phase_adjustment.cu:302-324):
+ * per frame, d_out[f] = the learned image d_img_learned[f] when its entropy d_ent_learned[f] is
+ * no worse than the plain result's d_ent_plain[f] (within rel_tol relative), else the plain
+ * image d_img_plain[f]; d_accepted[f] (optional) = 1 where the learned image was kept. Images
+ * are frames x n floats; d_out may alias either input. */
+ISAR_API int isar_accept_image(int frames, int n, const float* d_img_plain, const double* d_ent_plain,
+                               const float* d_img_learned, const double* d_ent_learned, double rel_tol, float* d_out,
+                               int32_t* d_accepted, void* stream);
 /* Half step from phi_a toward phi_b per pulse: phi_a sqrt(phi_b conj(phi_a)) on the shorter arc
  * (a second warm-start candidate, half of the predicted change). */
 ISAR_API int isar_half_phase(int frames, int M, const void* d_phi_a, const void* d_phi_b, void* d_out, void* stream);

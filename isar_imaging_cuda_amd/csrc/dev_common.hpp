@@ -58,6 +58,12 @@ __device__ __forceinline__ float2 expjpi_reduced(double t) {
     t -= 2.0 * rint(0.5 * t);
     return expjpi_twiddle(static_cast<float>(t));
 }
+// exp(j pi t) evaluated in FP64 and rounded once to FP32 (within half an ulp per component).
+__device__ __forceinline__ float2 expjpi_rounded(double t) {
+    double s, c;
+    sincospi(t, &s, &c);
+    return make_float2(static_cast<float>(c), static_cast<float>(s));
+}
 // exp(j pi t) for |t| small enough that FP32 holds it exactly (dyadic FFT twiddles).
 __device__ __forceinline__ float2 expjpi(float t) {
     float s, c;
@@ -340,6 +346,22 @@ __device__ __forceinline__ float wave_read_lane(float v, int lane) {
 #endif
 }
 
+// 1 / v to about 1 ulp (v_rcp_f32), without the correctly rounded division sequence.
+__device__ __forceinline__ float rcp_hw(float v) {
+#ifdef ISAR_EMU
+    return 1.0f / v;
+#else
+    return __builtin_amdgcn_rcpf(v);
+#endif
+}
+
+__device__ __forceinline__ float opaque(float v) {
+#ifndef ISAR_EMU
+    asm volatile("" : "+v"(v));
+#endif
+    return v;
+}
+
 __device__ __forceinline__ float2 opaque(float2 v) {
 #ifndef ISAR_EMU
     asm volatile("" : "+v"(v.x), "+v"(v.y));
@@ -556,9 +578,12 @@ __device__ __forceinline__ void fft_twiddles_fill(FftTwiddles<N, E>& tw, int tid
     if constexpr (P < Plan::NP) {
         constexpr int R = Plan::R[P];
         if constexpr (Ns > 1 && R == E) {
-            const float base = static_cast<float>(2 * (tid % Ns)) / static_cast<float>(Ns * R);
-            tw.w1[P] = expjpi_twiddle(base);
-            tw.w4[P] = R >= 8 ? expjpi_twiddle(4.0f * base) : make_float2(1.0f, 0.0f);
+            // exp(j pi b) and exp(j pi 4 b), b = 2k / (Ns R), from an FP64 sincospi rounded once:
+            // once per thread and kernel, so every transform of the kernel starts from bases
+            // within half an FP32 ulp (the powers are products of these)
+            const double base = static_cast<double>(2 * (tid % Ns)) / static_cast<double>(Ns * R);
+            tw.w1[P] = expjpi_rounded(base);
+            tw.w4[P] = R >= 8 ? expjpi_rounded(4.0 * base) : make_float2(1.0f, 0.0f);
         } else {
             tw.w1[P] = make_float2(1.0f, 0.0f);
             tw.w4[P] = make_float2(1.0f, 0.0f);

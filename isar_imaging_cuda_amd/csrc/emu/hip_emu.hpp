@@ -28,6 +28,7 @@
 #define HIP_SYMBOL(x) (&(x))
 
 struct float2 { float x, y; };
+struct float4 { float x, y, z, w; };
 struct double2 { double x, y; };
 struct int2 { int x, y; };
 inline float2 make_float2(float x, float y) { return float2{x, y}; }
@@ -94,6 +95,10 @@ inline void sincospif(float x, float* s, float* c) {
     const double a = 3.14159265358979323846 * static_cast<double>(x);
     *s = static_cast<float>(std::sin(a));
     *c = static_cast<float>(std::cos(a));
+}
+inline void sincospi(double x, double* s, double* c) {
+    *s = std::sin(3.14159265358979323846 * x);
+    *c = std::cos(3.14159265358979323846 * x);
 }
 inline float cospif(float x) { return static_cast<float>(std::cos(3.14159265358979323846 * static_cast<double>(x))); }
 inline float sinpif(float x) { return static_cast<float>(std::sin(3.14159265358979323846 * static_cast<double>(x))); }

@@ -249,6 +249,17 @@ ISAR_API int isar_half_phase(int frames, int M, const void* d_phi_a, const void*
     return hipGetLastError() == hipSuccess ? ISAR_OK : fail(ISAR_ERR_DEVICE, "isar_half_phase: launch failed");
 }
 
+ISAR_API int isar_accept_image(int frames, int n, const float* d_img_plain, const double* d_ent_plain,
+                               const float* d_img_learned, const double* d_ent_learned, double rel_tol, float* d_out,
+                               int32_t* d_accepted, void* stream) {
+    if (frames < 1 || n < 1 || !d_img_plain || !d_ent_plain || !d_img_learned || !d_ent_learned || !d_out ||
+        !(rel_tol >= 0.0))
+        return fail(ISAR_ERR_INVALID, "isar_accept_image: bad arguments");
+    isar::launch_accept_image(frames, n, d_img_plain, d_ent_plain, d_img_learned, d_ent_learned, rel_tol, d_out,
+                              d_accepted, static_cast<hipStream_t>(stream));
+    return hipGetLastError() == hipSuccess ? ISAR_OK : fail(ISAR_ERR_DEVICE, "isar_accept_image: launch failed");
+}
+
 ISAR_API int isar_image_entropy(int frames, int n, const float* d_img, double* d_out, void* stream) {
     if (frames < 1 || n < 1 || !d_img || !d_out) return fail(ISAR_ERR_INVALID, "isar_image_entropy: bad arguments");
     isar::launch_image_entropy(frames, n, d_img, d_out, static_cast<hipStream_t>(stream));

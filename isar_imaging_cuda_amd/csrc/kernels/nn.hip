@@ -1,5 +1,5 @@
 // nn.hip — kernels of the learned autofocus warm start: a small multilayer perceptron with BF16
-// weights and FP32 accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16), its backward
+// weights and FP32 accumulation on the matrix cores (gfx950's v_mfma_f32_16x16x32_bf16), its backward
 // pass and Adam update, and the per-pulse phase features it reads and writes.
 //
 // There is no counterpart in the reference: the network seeds fast-entropy autofocus
@@ -12,7 +12,7 @@ namespace isar {
 
 namespace {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bfloat16 bits, round to nearest even (finite inputs)
@@ -21,13 +21,14 @@ __device__ __forceinline__ short to_bf16(float f) {
     u += 0x7FFFu + ((u >> 16) & 1u);
     return static_cast<short>(u >> 16);
 }
-__device__ __forceinline__ short load_bf16(const float* p) { return to_bf16(*p); }
-__device__ __forceinline__ short load_bf16(const uint16_t* p) { return static_cast<short>(*p); }
+__device__ __forceinline__ __bf16 bits_bf16(short s) { return __builtin_bit_cast(__bf16, s); }
+__device__ __forceinline__ __bf16 load_bf16(const float* p) { return bits_bf16(to_bf16(*p)); }
+__device__ __forceinline__ __bf16 load_bf16(const uint16_t* p) { return bits_bf16(static_cast<short>(*p)); }
 
-// D = A B + C for one 16 x 16 x 16 tile. Lane l holds A[l % 16][4 (l / 16) + t],
-// B[4 (l / 16) + t][l % 16] and D[4 (l / 16) + t][l % 16], t < 4.
-__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+// D = A B + C for one 16 x 16 x 32 tile. Lane l holds A[l % 16][8 (l / 16) + t] and
+// B[8 (l / 16) + t][l % 16] for t < 8, and D[4 (l / 16) + t][l % 16] for t < 4.
+__device__ __forceinline__ f32x4 mfma16x32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
 constexpr int kGemmTile = 32;   // output rows and columns per wave (2 x 2 MFMA tiles)
@@ -44,36 +45,37 @@ __global__ __launch_bounds__(64) void k_gemm_bf16(int m, int n, int k, const TA*
                                                   const float* __restrict__ mask, long smm) {
     const int l = threadIdx.x;
     const int bm = blockIdx.y * kGemmTile, bn = blockIdx.x * kGemmTile;
-    const int li = l % 16, lk = 4 * (l / 16);
+    const int li = l % 16, lk = 8 * (l / 16), lr = 4 * (l / 16);
     f32x4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < k; k0 += 16) {
-        bf16x4 fa[2], fb[2];
+    const __bf16 zero = bits_bf16(0);
+    for (int k0 = 0; k0 < k; k0 += 32) {
+        bf16x8 fa[2], fb[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
             const int row = bm + 16 * i + li;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < 8; ++t) {
                 const int kk = k0 + lk + t;
-                fa[i][t] = (row < m && kk < k) ? load_bf16(a + row * sam + kk * sak) : static_cast<short>(0);
+                fa[i][t] = (row < m && kk < k) ? load_bf16(a + row * sam + kk * sak) : zero;
             }
         }
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const int col = bn + 16 * j + li;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
+            for (int t = 0; t < 8; ++t) {
                 const int kk = k0 + lk + t;
-                fb[j][t] = (col < n && kk < k) ? load_bf16(b + kk * sbk + col * sbn) : static_cast<short>(0);
+                fb[j][t] = (col < n && kk < k) ? load_bf16(b + kk * sbk + col * sbn) : zero;
             }
         }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = mfma16(fa[i], fb[j], acc[i][j]);
+            for (int j = 0; j < 2; ++j) acc[i][j] = mfma16x32(fa[i], fb[j], acc[i][j]);
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -83,7 +85,7 @@ __global__ __launch_bounds__(64) void k_gemm_bf16(int m, int n, int k, const TA*
             if (col >= n) continue;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const int row = bm + 16 * i + lk + t;
+                const int row = bm + 16 * i + lr + t;
                 if (row >= m) continue;
                 float v = acc[i][j][t];
                 if constexpr (EPI == 1) v = fmaxf(v + bias[col], 0.0f);
@@ -187,6 +189,20 @@ __global__ void k_select_phase(int frames, int M, const float2* __restrict__ phi
     if (ent_out && i % M == 0) ent_out[f] = take_b ? ent_b[f] : ent_a[f];
 }
 
+// The final judgement of a learned result: per frame, out = img_b (the learned image) when its
+// entropy is no worse than img_a's (the plain result) within rel_tol, else img_a; accepted[f]
+// records the choice. out may alias either input.
+__global__ void k_accept_image(int frames, int n, const float* __restrict__ img_a, const double* __restrict__ ent_a,
+                               const float* img_b, const double* __restrict__ ent_b, double rel_tol, float* out,
+                               int32_t* __restrict__ accepted) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= static_cast<long long>(frames) * n) return;
+    const int f = static_cast<int>(i / n);
+    const bool take_b = ent_b[f] <= ent_a[f] * (1.0 + rel_tol);
+    out[i] = take_b ? img_b[i] : img_a[i];
+    if (accepted && i % n == 0) accepted[f] = take_b ? 1 : 0;
+}
+
 // Half step from phi_a toward phi_b: out = phi_a * sqrt(unit(phi_b conj(phi_a))), the square root
 // taken on the shorter arc (unit(1 + c)); an exactly opposite pair turns by +pi/2.
 __global__ void k_half_phase(int frames, int M, const float2* __restrict__ phi_a, const float2* __restrict__ phi_b,
@@ -286,6 +302,12 @@ void launch_select_phase(int frames, int M, const float2* phi_a, const double* e
 }
 void launch_half_phase(int frames, int M, const float2* phi_a, const float2* phi_b, float2* out, hipStream_t s) {
     ISAR_LAUNCH(k_half_phase, dim3((frames * M + 255) / 256), dim3(256), 0, s, frames, M, phi_a, phi_b, out);
+}
+void launch_accept_image(int frames, int n, const float* img_a, const double* ent_a, const float* img_b,
+                         const double* ent_b, double rel_tol, float* out, int32_t* accepted, hipStream_t s) {
+    const long long total = static_cast<long long>(frames) * n;
+    ISAR_LAUNCH(k_accept_image, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, s, frames, n, img_a,
+                ent_a, img_b, ent_b, rel_tol, out, accepted);
 }
 void launch_image_entropy(int frames, int n, const float* img, double* out, hipStream_t s) {
     ISAR_LAUNCH(k_image_entropy, dim3(frames), dim3(256), 0, s, n, img, out);

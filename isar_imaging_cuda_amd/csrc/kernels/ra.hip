@@ -141,6 +141,16 @@ __device__ int correlation_argmax(const float (&a)[kE], const float (&b)[kE], fl
     return bi;
 }
 
+// sqrt(x) to about 1e-14 relative for x >= 0: the hardware FP32 root (within 1 ulp) refined by one
+// Newton step in FP64, s + (x - s^2) / (2 s); the step's own factor 1 / (2 s) needs only the
+// hardware reciprocal's accuracy (1 ulp). A few FP64 operations instead of the long correctly rounded FP64 square root.
+__device__ __forceinline__ double sqrt_refined(float x) {
+    const float s = sqrt_hw(x);
+    const double sd = static_cast<double>(s);
+    const double d = fma(-sd, sd, static_cast<double>(x));
+    return s > 0.0f ? fma(d, static_cast<double>(0.5f * rcp_hw(s)), sd) : 0.0;
+}
+
 // Parabolic lag around the argmax bi (binomialFixDevice, range_alignment.cu:121-131). The three
 // samples c[bi-1], c[bi], c[bi+1] are formed again as direct sums sum_n gA[n] gB[n - l] in FP64,
 // not read from the FP32 transform: near a broad peak the second
@@ -159,20 +169,22 @@ __device__ double parabola_lag(const float (&a)[kE], const float (&b)[kE], int b
     // doubles fit the transform's padded N float2); each thread sums over its own cells. The
     // per-wave sums follow gB in LDS and are added in wave order, so the result does not depend
     // on scheduling.
-    // The profiles are the FP64 square roots of the FP32 energies (the hardware FP32 root is
-    // off by up to 1 ulp, which the vertex would see).
+    // The profiles are the square roots of the FP32 energies to FP64 accuracy (the hardware FP32
+    // root alone is off by up to 1 ulp, which the vertex would see). The energies pass through
+    // opaque() so that the roots are not shared with the transform input above and kept live
+    // across its transforms.
     double* gb = reinterpret_cast<double*>(lds);
     double* red = gb + N;
     auto pos = [&](int e) { return OUT ? out_pos<N>(tid, e) : in_pos<N>(tid, e); };
 #pragma unroll
-    for (int e = 0; e < kE; ++e) gb[pos(e)] = sqrt(static_cast<double>(b[e]));
+    for (int e = 0; e < kE; ++e) gb[pos(e)] = sqrt_refined(opaque(b[e]));
     __syncthreads();
     const int l0 = bi - N / 2;
     double s1 = 0.0, s2 = 0.0, s3 = 0.0;
 #pragma unroll
     for (int e = 0; e < kE; ++e) {
         const int n = pos(e);
-        const double ga = sqrt(static_cast<double>(a[e]));
+        const double ga = sqrt_refined(opaque(a[e]));
         s1 = fma(ga, gb[(n - l0 + 1) & (N - 1)], s1);
         s2 = fma(ga, gb[(n - l0) & (N - 1)], s2);
         s3 = fma(ga, gb[(n - l0 - 1) & (N - 1)], s3);

@@ -64,6 +64,8 @@ void launch_image_entropy(int frames, int n, const float* img, double* out, hipS
 void launch_select_phase(int frames, int M, const float2* phi_a, const double* ent_a, const float2* phi_b,
                          const double* ent_b, float2* out, double* ent_out, hipStream_t s);
 void launch_half_phase(int frames, int M, const float2* phi_a, const float2* phi_b, float2* out, hipStream_t s);
+void launch_accept_image(int frames, int n, const float* img_a, const double* ent_a, const float* img_b,
+                         const double* ent_b, double rel_tol, float* out, int32_t* accepted, hipStream_t s);
 
 // standalone batched FFT used by the tests: n-point transforms of `batch` contiguous rows
 bool debug_fft(int n, int dir, const float2* in, float2* out, int batch, hipStream_t s);

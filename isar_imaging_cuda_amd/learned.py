@@ -71,6 +71,7 @@ def _bind() -> C.CDLL:
     L.isar_image_entropy.argtypes = [i, i, vp, vp, vp]
     L.isar_select_phase.argtypes = [i, i, vp, vp, vp, vp, vp, vp, vp]
     L.isar_half_phase.argtypes = [i, i, vp, vp, vp, vp]
+    L.isar_accept_image.argtypes = [i, i, vp, vp, vp, vp, C.c_double, vp, vp, vp]
     L._learned_bound = True
     return L
 
@@ -159,6 +160,14 @@ def half_phase(frames: int, M: int, d_phi_a: int, d_phi_b: int, d_out: int, stre
     _check(_bind().isar_half_phase(frames, M, _p(d_phi_a), _p(d_phi_b), _p(d_out), _p(stream)), "isar_half_phase")
 
 
+def accept_image(frames: int, n: int, d_img_plain: int, d_ent_plain: int, d_img_learned: int, d_ent_learned: int,
+                 d_out: int, d_accepted: int | None = None, rel_tol: float = ENTROPY_REL,
+                 stream: int | None = None) -> None:
+    _check(_bind().isar_accept_image(frames, n, _p(d_img_plain), _p(d_ent_plain), _p(d_img_learned),
+                                     _p(d_ent_learned), rel_tol, _p(d_out), _p(d_accepted), _p(stream)),
+           "isar_accept_image")
+
+
 # ------------------------------------------------------------------------------------------
 # the pipeline with a learned warm start
 # ------------------------------------------------------------------------------------------
@@ -241,12 +250,33 @@ class WarmStartImager:
         select_phase(K, self.M, self.phi_t.data_ptr(), self.ent_t.data_ptr(), self.phi_h.data_ptr(),
                      self.ent_half.data_ptr(), self.phi_s.data_ptr(), stream, self.ent_start.data_ptr())
 
-    def run(self, K: int, echo_ptr: int, vel_ptr: int, img_ptr: int, iterations: int, stream: int) -> None:
-        """The learned chain: F + `iterations` autofocus iterations in total."""
+    def run(self, K: int, echo_ptr: int, vel_ptr: int, img_ptr: int, iterations: int, stream: int,
+            final_judge: bool = True):
+        """The learned chain: F + `iterations` autofocus iterations in total. With final_judge
+        (the default) the plain 120-iteration result is formed too and remains the judge: a
+        frame keeps its learned image only when that image's entropy is no worse than the plain
+        image's (within 1e-5 relative), else it gets the plain image. Returns the per-frame
+        acceptance flags (a K-element int32 device tensor) or None without the judgement."""
         self.front(K, echo_ptr, vel_ptr, stream)
         self.predict(K, stream)
         self.judge(K, img_ptr, stream)
         self.back(K, img_ptr, iterations, stream)
+        if not final_judge:
+            return None
+        torch = self.torch
+        if getattr(self, "img_plain", None) is None:
+            dev = self.phis[0].device
+            self.img_plain = torch.zeros((self.K, self.M, self.W), dtype=torch.float32, device=dev)
+            self.ent_final_plain = torch.zeros(self.K, dtype=torch.float64, device=dev)
+            self.ent_final_learned = torch.zeros(self.K, dtype=torch.float64, device=dev)
+            self.accepted = torch.zeros(self.K, dtype=torch.int32, device=dev)
+        n = self.M * self.W
+        image_entropy(K, n, img_ptr, self.ent_final_learned.data_ptr(), stream)
+        self.back(K, self.img_plain.data_ptr(), 120, stream, warm=False)
+        image_entropy(K, n, self.img_plain.data_ptr(), self.ent_final_plain.data_ptr(), stream)
+        accept_image(K, n, self.img_plain.data_ptr(), self.ent_final_plain.data_ptr(), img_ptr,
+                     self.ent_final_learned.data_ptr(), img_ptr, self.accepted.data_ptr(), stream=stream)
+        return self.accepted[:K]
 
 
 # ------------------------------------------------------------------------------------------
@@ -396,6 +426,45 @@ def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int
             "plain_median": float(np.median(ip)), "learned_median": float(np.median(il)),
             "entropy_rel_diff_max": float(np.max((np.concatenate(e_learned_all) - np.concatenate(e_plain_all))
                                                  / np.concatenate(e_plain_all)))}
+
+
+def stage_times(params: np.ndarray, seeds: list[int], config: int = 4, front_iters: int = FRONT_ITERS,
+                repeats: int = 5) -> dict:
+    """Device time per frame (ms, HIP events on the run stream, one batch of len(seeds) frames, the
+    median of `repeats`) of the pieces the two chains are made of: one autofocus iteration, the
+    image formation after the autofocus (MTRC, azimuth FFT), the network (features, forward,
+    composed start) and the judgement of the start (three image formations, their entropies
+    and two selections). Range alignment, centring, cut and bin selection are common to both
+    chains and left out."""
+    import torch
+    dev = torch.device("cuda:0")
+    echo, vel, h = synth_frames(config, seeds)
+    K, M, N = echo.shape
+    ws = WarmStartImager(_radar(h, M, N), K, params, front_iters=front_iters)
+    img = torch.zeros((K, M, ws.W), dtype=torch.float32, device=dev)
+    e = torch.from_numpy(echo.view(np.float32)).to(dev)
+    v = torch.from_numpy(vel).to(dev)
+    stream = torch.cuda.current_stream(dev)
+    s = stream.cuda_stream
+    ws.front(K, e.data_ptr(), v.data_ptr(), s)
+
+    def timed(fn) -> float:
+        out = []
+        for _ in range(repeats + 1):
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            out.append(a.elapsed_time(b))
+        return float(np.median(out[1:]))
+
+    t0 = timed(lambda: ws.back(K, img.data_ptr(), 0, s, warm=False))
+    t120 = timed(lambda: ws.back(K, img.data_ptr(), 120, s, warm=False))
+    t_net = timed(lambda: ws.predict(K, s))
+    t_judge = timed(lambda: ws.judge(K, img.data_ptr(), s))
+    return {"iteration": (t120 - t0) / 120 / K, "image": t0 / K, "network": t_net / K, "judge": t_judge / K,
+            "batch_frames": K}
 
 
 def load_weights(path: Path | str | None = None, M: int = 256) -> tuple[np.ndarray, int]:

@@ -252,17 +252,59 @@ def test_training_on_pipeline_frames_reduces_loss():
 
 @pytest.mark.gpu
 def test_learned_warm_start_needs_fewer_iterations():
-    """The shipped network against the plain iteration on held-out frames: the learned chain
-    reaches the plain 120-iteration image entropy (within 1e-5 relative) in fewer iterations."""
+    """The shipped network against the plain iteration on 256 held-out frames: the learned chain
+    reaches the plain 120-iteration image entropy (within 1e-5 relative) on every frame, in at
+    least 2x fewer iterations on average (its count includes the two front iterations)."""
     _torch()
     from isar_imaging_cuda_amd import learned as L
     try:
         params, front_iters = L.load_weights()
     except Exception as e:
         pytest.skip(str(e))
-    print("[case] learned vs plain, 64 held-out frames", flush=True)
-    res = L.evaluate(params, list(range(20_000_001, 20_000_065)), chunk=64, log=lambda m: None,
+    print("[case] learned vs plain, 256 held-out frames", flush=True)
+    res = L.evaluate(params, list(range(20_000_001, 20_000_257)), chunk=256, log=lambda m: None,
                      front_iters=front_iters)
     print(res)
-    assert res["accepted_fraction"] >= 0.9
-    assert res["learned_mean_iterations"] < res["plain_mean_iterations"]
+    assert res["frames"] == 256
+    assert res["accepted_fraction"] == 1.0
+    assert res["iteration_ratio"] >= 2.0
+    assert res["entropy_rel_diff_max"] <= L.ENTROPY_REL
+
+
+@pytest.mark.gpu
+def test_final_judge_falls_back_to_the_plain_image():
+    """The product path keeps a learned image only when its entropy is no worse than the plain
+    120-iteration image's (phase_adjustment.cu:302-324 is the judge). An untrained network with
+    3 iterations after the judged start leaves frames short of the plain entropy: those frames
+    get the plain image bit for bit, the others keep the learned image."""
+    torch = _torch()
+    import isar_imaging_cuda_amd as isar
+    from isar_imaging_cuda_amd import learned as L
+    K = 32
+    echo, vel, h = _c4_batch(K, 700)
+    M, N = echo.shape[1:]
+    print(f"[case] final entropy judge, {K} frames {M}x{N}", flush=True)
+    params = isar.RadarParams(M, N, h["band_width"], h["fc"], h["Tp"], h["Fs"])
+    dev = torch.device("cuda:0")
+    e = torch.from_numpy(echo.view(np.float32)).to(dev)
+    v = torch.from_numpy(vel).to(dev)
+    s = torch.cuda.current_stream().cuda_stream
+    ws = L.WarmStartImager(params, K, L.he_init(L.widths_for(M), seed=3))
+    learned = torch.zeros((K, M, 512), dtype=torch.float32, device=dev)
+    ws.run(K, e.data_ptr(), v.data_ptr(), learned.data_ptr(), 3, s, final_judge=False)
+    out = torch.zeros_like(learned)
+    acc = ws.run(K, e.data_ptr(), v.data_ptr(), out.data_ptr(), 3, s).cpu().numpy()
+    plain = torch.zeros_like(learned)
+    ws.imager.run_device(e.data_ptr(), v.data_ptr(), plain.data_ptr(), K, isar.Options(), stream=s)
+    torch.cuda.synchronize()
+    from oracle import oracle as O
+    rejected = 0
+    for k in range(K):
+        el, ep = O.image_entropy(learned[k].cpu().numpy()), O.image_entropy(plain[k].cpu().numpy())
+        if acc[k]:
+            assert torch.equal(out[k], learned[k]) and el <= ep * (1 + L.ENTROPY_REL) + 1e-12, k
+        else:
+            rejected += 1
+            assert torch.equal(out[k], plain[k]) and el > ep * (1 + L.ENTROPY_REL) - 1e-12, k
+    print(f"{rejected} of {K} frames fell back to the plain image")
+    assert rejected > 0

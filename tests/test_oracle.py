@@ -442,9 +442,9 @@ def test_fp32_range_profiles_alone_move_the_large_frame_by_about_the_bar():
     scatterers, 7 km/s): the oracle with only its range-alignment row transforms in FP32 (radix
     2, correctly rounded twiddles; every other step FP64) against itself in FP64. The lags then
     differ by up to a few 1e-6 cell, and 120 autofocus iterations turn the resulting per-pulse
-    phase differences into an image difference of order 1e-4, the parity bar itself: an FP32
-    pipeline cannot be held to 1e-4 on this frame, and its bar is set from this figure. The
-    integer decisions do not change."""
+    phase differences into an image difference of order 1e-4, the parity bar itself: on this
+    frame an FP32 pipeline meets the bar only with transforms about as accurate as these (FP64
+    lag vertex, twiddles within half an ulp). The integer decisions do not change."""
     cfg = synth.preset(5)
     echo, vel = synth.frame(cfg)
     h = synth.radar_header(cfg)
