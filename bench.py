@@ -76,9 +76,23 @@ def stage_bytes_per_frame(stage: str, M: int, N: int, W: int, U: float) -> float
     }[stage]
 
 
-def cpu_baseline(frames: int = 64, runs: int = 3) -> dict:
-    """FP64 oracle with OpenMP over frames on this host (bounded sample of the same workload);
-    frames/s of the median of `runs` timed runs."""
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), as lscpu reports it."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(frames: int = 64, runs: int = 3, intra_frames: int = 8) -> dict:
+    """FP64 oracle on this host's cores (bounded sample of the same workload), with the two ways
+    to spread it over threads: frame-level (one OpenMP thread per frame, `frames` frames, median
+    of `runs` timed runs) and intra-frame (every thread inside each stage of one frame,
+    `intra_frames` frames in turn). `value` is the faster; the intra-frame runs give the time of
+    each of the reference's SEPARATE_TIMING_ stages (isar_main.cu:34-218)."""
     from isar_imaging_cuda_amd import synth
     from oracle import oracle as O
 
@@ -95,9 +109,52 @@ def cpu_baseline(frames: int = 64, runs: int = 3) -> dict:
         _, threads = O.isar_batch(echo, vel, p)
         walls.append(time.perf_counter() - t0)
     dt = float(np.median(walls))
-    return {"value": frames / dt, "unit": "frames/s", "cores": int(threads), "kind": "port",
-            "sample": f"{frames} distinct 256x4096 frames, FP64 oracle, one OpenMP thread per frame (frame-level); "
-                      f"median of {runs} runs, {dt:.1f} s wall each"}
+    frame_level = frames / dt
+    stages, t_intra = [], []
+    for k in range(intra_frames):
+        t0 = time.perf_counter()
+        _, st = O.isar_stage_times(echo[k], vel[k], p)
+        t_intra.append(time.perf_counter() - t0)
+        stages.append(st)
+    intra = intra_frames / float(np.sum(t_intra))
+    stage_ms = {name: round(float(np.median([s[name] for s in stages])), 2) for name in O.STAGE_NAMES}
+    best = "frame-level" if frame_level >= intra else "intra-frame"
+    return {"value": max(frame_level, intra), "unit": "frames/s", "cores": int(threads), "kind": "port",
+            "cpu_model": cpu_model(), "parallelism": best,
+            "frame_level_frames_per_s": round(frame_level, 3), "intra_frame_frames_per_s": round(intra, 3),
+            "stage_ms_per_frame_intra": stage_ms,
+            "sample": f"{frames} distinct 256x4096 frames, FP64 oracle: frame-level one OpenMP thread per frame "
+                      f"(median of {runs} runs, {dt:.1f} s wall each) and intra-frame all threads per stage "
+                      f"({intra_frames} frames, {float(np.sum(t_intra)):.1f} s); value = the faster ({best})"}
+
+
+# the reference's SEPARATE_TIMING_ stages (isar_main.cu:34-218) and the device kernel families
+# that carry their work; HPC runs inside the first merge level (k_ra_pair0), and the MTRC's last
+# row transform is done together with post-processing (k_mtrc_image)
+GPU_STAGES = {"hpc": (), "range_alignment": ("ra_pair0", "ra_corr", "ra_upd"), "centre_cut": ("center", "cut"),
+              "phase_compensation": ("pa_colfft", "pa_select", "autofocus"), "mtrc": ("mtrc_rows", "mtrc_czt"),
+              "post": ("mtrc_image",)}
+
+
+def gpu_stage_times(im, step, frames: int, steps: int = 3) -> dict:
+    """Device time per frame (us) of each SEPARATE_TIMING_ stage: HIP events around every launch
+    of the stage's kernel families on the run stream, `steps` steps per family."""
+    import torch
+
+    import isar_imaging_cuda_amd as isar
+    out = {}
+    for stage, fams in GPU_STAGES.items():
+        ms = 0.0
+        for fam in fams:
+            im.probe_enable(isar.STAGES[fam])
+            for _ in range(steps):
+                step()
+            torch.cuda.synchronize()
+            t, _ = im.probe_read()
+            ms += t / steps
+        im.probe_enable(-1)
+        out[stage] = round(ms * 1e3 / frames, 3)
+    return out
 
 
 def copy_calibration(mib: int) -> int:
@@ -137,9 +194,11 @@ def learned_vs_plain(frames: int = 1000) -> dict:
     return res
 
 
-def subapertures(count: int = 64, replays: int = 10) -> dict:
+def subapertures(count: int = 64, replays: int = 10, autotune: bool = True) -> dict:
     """64 sliding windows of {128, 192, 256, 384, 512} pulses of one 8192 x 4096 complex64 dwell
-    imaged by one sub-aperture plan (graph replay after autotuning)."""
+    imaged by one sub-aperture plan (graph replay after autotuning; without it, the default
+    chunk size and autofocus form, so that a kernel trace holds only the capture run and the
+    replays)."""
     import torch
 
     import isar_imaging_cuda_amd as isar
@@ -156,7 +215,8 @@ def subapertures(count: int = 64, replays: int = 10) -> dict:
     d_echo = torch.from_numpy(echo.view(np.float32)).to(dev)
     d_vel = torch.from_numpy(vel).to(dev)
     img = torch.zeros(plan.image_floats, dtype=torch.float32, device=dev)
-    chosen = plan.autotune(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), repeats=2)
+    chosen = (plan.autotune(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), repeats=2) if autotune
+              else np.full(len(plan.group_pulses), 8))
     s = torch.cuda.current_stream(dev)
     plan.run(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), graph=True, stream=s.cuda_stream)
     torch.cuda.synchronize()
@@ -308,6 +368,9 @@ def main() -> int:
     ap.add_argument("--subapertures-only", action="store_true", help="only the sub-aperture plan (for profiling)")
     ap.add_argument("--no-transfer", action="store_true", help="skip the host-transfer (PCIe) timing")
     ap.add_argument("--no-single", action="store_true", help="skip the single-frame latency")
+    ap.add_argument("--no-stages", action="store_true", help="skip the per-stage device times")
+    ap.add_argument("--sa-fixed", action="store_true",
+                    help="sub-aperture plan without autotuning (default chunk size and autofocus form)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -321,7 +384,7 @@ def main() -> int:
         print(json.dumps({"large_batch": large_batch()}))
         return 0
     if args.subapertures_only:
-        print(json.dumps({"subapertures": subapertures()}))
+        print(json.dumps({"subapertures": subapertures(autotune=not args.sa_fixed)}))
         return 0
 
     import torch
@@ -389,6 +452,7 @@ def main() -> int:
     probe_ms, probe_n = im.probe_read()
     im.probe_enable(-1)
     avg_launch_ms = probe_ms / max(probe_n, 1)
+    stage_us = gpu_stage_times(im, step, K) if not args.no_stages else None
     alg = stage_bytes_per_frame(probe, M, N, W, U) * K
     achieved_gbs = alg / (avg_launch_ms / 1e3) / 1e9 if probe_n else 0.0
     traffic = load_traffic(probe, K)
@@ -422,6 +486,10 @@ def main() -> int:
                               "achieved_GBs": round(frame_bytes * fps / 1e9, 2),
                               "frac": round(frame_bytes * fps / 1e9 / HBM_PEAK_GBS, 5)},
     }
+    if stage_us is not None:
+        out["stage_us_per_frame"] = stage_us
+        out["stage_note"] = ("HIP-event time per frame of the kernels that do each SEPARATE_TIMING_ stage; HPC is fused "
+                             "into the first range-alignment kernel, the MTRC's last row FFT into post-processing")
     if not args.no_transfer:
         out["host_transfer"] = host_transfer(echo, img, ms_per_step, stream)
     if not args.no_single:

@@ -24,6 +24,7 @@
 #include "isar_oracle.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -738,13 +739,27 @@ int orc_hrrp(const float* echo, const double* velocity, const orc_params* p, con
 
 int orc_isar_main(const float* echo, const double* velocity, const orc_params* p,
                   const orc_options* opt, float* img, orc_trace* tr) {
-    // isar_main.cu:3-227 (isarMainSingle, isar_imaging_export.cpp:208-212)
+    return orc_isar_stage_times(echo, velocity, p, opt, img, tr, nullptr);
+}
+
+int orc_isar_stage_times(const float* echo, const double* velocity, const orc_params* p,
+                         const orc_options* opt, float* img, orc_trace* tr, double* stage_ms) {
+    // isar_main.cu:3-227 (isarMainSingle, isar_imaging_export.cpp:208-212); stage_ms (optional)
+    // gets the wall time of the reference's SEPARATE_TIMING_ stages (isar_main.cu:34-218)
     const int M = p->echo_num, N = p->range_num, W = p->range_num_cut;
     // imagingMemInit validation (isar_imaging_export.cpp:144-155)
     if (M < 2 || M > 1024 || N < W || N < M || W < 2 || !is_pow2(N)) return 1;
     const size_t MN = static_cast<size_t>(M) * N, MW = static_cast<size_t>(M) * W;
+    auto clock = [] { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
+    double t0 = clock();
+    auto lap = [&](int stage) {
+        const double t = clock();
+        if (stage_ms) stage_ms[stage] = t - t0;
+        t0 = t;
+    };
     std::vector<cd> S(MN);
     front_rows(S, echo, velocity, p, opt);
+    lap(0);
     if (opt->if_hrrp && opt->data_type == 2) {
         // getHRRP, common.cu:563-566: the STRETCH branch windows d_data in place (Q11 kept);
         // the HRRP itself is a side output that the image does not depend on.
@@ -755,12 +770,17 @@ int orc_isar_main(const float* echo, const double* velocity, const orc_params* p
     t_range_fft32 = opt->range_fft32;
     orc_range_alignment(reinterpret_cast<double*>(S.data()), p, tr);
     t_range_fft32 = 0;
+    lap(1);
     orc_hrrp_center(reinterpret_cast<double*>(S.data()), p, tr);
     std::vector<cd> X(MW);
     orc_cut_range_profile(reinterpret_cast<const double*>(S.data()), p, reinterpret_cast<double*>(X.data()), tr);
+    lap(2);
     orc_fast_entropy(reinterpret_cast<double*>(X.data()), M, W, opt->iterations, opt->phi0_given, tr);
+    lap(3);
     if (opt->if_mtrc) orc_mtrc(reinterpret_cast<double*>(X.data()), p, opt->data_type);
+    lap(4);
     orc_post_process(reinterpret_cast<double*>(X.data()), M, W, opt->quirk_q1, opt->data_type, img);
+    lap(5);
     if (tr) tr->entropy = orc_image_entropy(img, static_cast<int>(MW));
     return 0;
 }

@@ -98,6 +98,13 @@ typedef struct orc_trace {
 int orc_isar_main(const float* echo, const double* velocity, const orc_params* p,
                   const orc_options* opt, float* img, orc_trace* trace);
 
+/* orc_isar_main with the wall time (ms) of the reference's SEPARATE_TIMING_ stages
+ * (isar_main.cu:34-218) in stage_ms[6] (may be NULL): 0 HPC (with the IFDS pre-stage), 1 range
+ * alignment, 2 HRRP centring and cut, 3 phase compensation (Doppler tracking, bin selection,
+ * fast entropy), 4 MTRC, 5 post-processing (window, azimuth FFT, |.|). */
+int orc_isar_stage_times(const float* echo, const double* velocity, const orc_params* p,
+                         const orc_options* opt, float* img, orc_trace* trace, double* stage_ms);
+
 /* K frames in parallel (one OpenMP thread per frame). echo: K*M*N complex, velocity: K*M,
  * img: K*M*W. Traces are not collected. Returns the number of threads used. */
 int orc_isar_batch(int K, const float* echo, const double* velocity, const orc_params* p,

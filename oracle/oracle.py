@@ -164,6 +164,27 @@ def isar_main(echo: np.ndarray, velocity: np.ndarray, p: OrcParams, opt: OrcOpti
     return img
 
 
+STAGE_NAMES = ("hpc", "range_alignment", "centre_cut", "phase_compensation", "mtrc", "post")
+
+
+def isar_stage_times(echo: np.ndarray, velocity: np.ndarray, p: OrcParams, opt: OrcOptions | None = None
+                     ) -> tuple[np.ndarray, dict]:
+    """One frame through the oracle with the wall time (ms) of each SEPARATE_TIMING_ stage
+    (isar_main.cu:34-218); the stages use every OpenMP thread (intra-frame parallelism)."""
+    opt = opt or options()
+    M, W = p.echo_num, p.range_num_cut
+    e = np.ascontiguousarray(echo, dtype=np.complex64)
+    v = np.ascontiguousarray(velocity, dtype=np.float64)
+    img = np.zeros((M, W), np.float32)
+    ms = np.zeros(6, np.float64)
+    L = lib()
+    L.orc_isar_stage_times.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(OrcParams), C.POINTER(OrcOptions),
+                                       C.c_void_p, C.c_void_p, C.c_void_p]
+    if L.orc_isar_stage_times(_ptr(e), _ptr(v), C.byref(p), C.byref(opt), _ptr(img), None, _ptr(ms)) != 0:
+        raise ValueError("oracle rejected the frame shape")
+    return img, dict(zip(STAGE_NAMES, ms.tolist()))
+
+
 def isar_batch(echo: np.ndarray, velocity: np.ndarray, p: OrcParams, opt: OrcOptions | None = None) -> tuple[np.ndarray, int]:
     opt = opt or options()
     K = echo.shape[0]
