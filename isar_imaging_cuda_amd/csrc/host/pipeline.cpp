@@ -104,6 +104,7 @@ size_t carve_workspace(const isar_radar_params& p, int K_, char* base, Job* J, S
     j.dec = static_cast<FrameDecisions*>(frame(sizeof(FrameDecisions)));
     j.x = static_cast<float2*>(frame(M * W * sizeof(float2)));
     j.rowdop = static_cast<float2*>(frame(M * sizeof(float2)));
+    j.dopphi = static_cast<float2*>(frame(M * sizeof(float2)));
     j.colmax_part = static_cast<float*>(frame(nbands * W * sizeof(float)));
     j.colmax = static_cast<float*>(frame(W * sizeof(float)));
     j.colE = static_cast<float*>(frame(W * sizeof(float)));

@@ -179,11 +179,12 @@ __global__ __launch_bounds__(kCenterThreads) void k_center(Job J) {
 
 // Cut: X[m, j] = S[m, (off + j - shift) mod N] for a band of rows; column maxima of |X| per
 // band (fastEntropy :180-184) and the adjacent-pulse products sum_j conj(X[m,j]) X[m+1,j]
-// (dopplerTracking :12-20). Block = W threads, one per range bin of the window.
+// (dopplerTracking :12-20). Block = W threads, one per range bin of the window. Each wave's sum
+// of a row's products goes to LDS; one barrier after the band, then thread r adds row r's wave
+// sums in wave order.
 template <int W>
 __global__ __launch_bounds__(W) void k_cut(Job J) {
-    HIP_DYNAMIC_SHARED(float, sm);
-    float* red = sm;   // 2 * (W/64) floats
+    HIP_DYNAMIC_SHARED(float2, wred);   // (W/64) x kCutBandRows wave sums
     const int band = blockIdx.x, f = blockIdx.y, j = threadIdx.x;
     const int M = J.M, N = J.N;
     const int nbands = (M + kCutBandRows - 1) / kCutBandRows;
@@ -203,20 +204,21 @@ __global__ __launch_bounds__(W) void k_cut(Job J) {
             nxt = src[static_cast<size_t>(m + 1) * N];
             prod = cconjmul(cur, nxt);
         }
-        // workgroup sum of the products for row m
-        float2 sum = wave_sum2(prod);
-        const int w = j >> 6, l = j & 63;
-        __syncthreads();
-        if (l == 0) { red[2 * w] = sum.x; red[2 * w + 1] = sum.y; }
-        __syncthreads();
-        if (j == 0 && m + 1 < M) {
-            float2 t = make_float2(0.0f, 0.0f);
-            for (int k = 0; k < W / 64; ++k) { t.x += red[2 * k]; t.y += red[2 * k + 1]; }
-            J.rowdop[static_cast<size_t>(f) * M + m] = t;
-        }
+        const float2 sum = wave_sum2(prod);
+        if ((j & 63) == 0) wred[(j >> 6) * kCutBandRows + (m - m0)] = sum;
         cur = nxt;
     }
     J.colmax_part[(static_cast<size_t>(f) * nbands + band) * W + j] = cmax;
+    __syncthreads();
+    if (j < m1 - m0 && m0 + j + 1 < M) {
+        float2 t = make_float2(0.0f, 0.0f);
+        for (int k = 0; k < W / 64; ++k) {
+            const float2 v = wred[k * kCutBandRows + j];
+            t.x += v.x;
+            t.y += v.y;
+        }
+        J.rowdop[static_cast<size_t>(f) * M + m0 + j] = t;
+    }
 }
 
 hipError_t configure_center_kernels() {
@@ -241,7 +243,8 @@ void launch_center(const Job& J, hipStream_t s) {
 void launch_cut(const Job& J, hipStream_t s) {
     const int nbands = (J.M + kCutBandRows - 1) / kCutBandRows;
     if (J.W == kRangeCut)
-        ISAR_LAUNCH(k_cut<kRangeCut>, dim3(nbands, J.K), dim3(kRangeCut), 64 * sizeof(float), s, J);
+        ISAR_LAUNCH(k_cut<kRangeCut>, dim3(nbands, J.K), dim3(kRangeCut), (kRangeCut / 64) * kCutBandRows * sizeof(float2),
+                    s, J);
 }
 
 }  // namespace isar

@@ -46,6 +46,16 @@ __device__ void bitonic_sort_kv(float* key, int* tag, int n) {
 
 }  // namespace
 
+// Doppler-centroid phase of the cut (dopplerTracking, phase_adjustment.cu:4-41) from the
+// adjacent-pulse products k_cut formed: once per frame, for every column tile of k_pa_colfft.
+template <int M>
+__global__ __launch_bounds__(256) void k_pa_doppler(Job J) {
+    HIP_DYNAMIC_SHARED(double, scr);   // M (scan) + 256 (scan partials)
+    const int f = blockIdx.x;
+    doppler_phase<256>(J.rowdop + static_cast<size_t>(f) * M, M, J.dopphi + static_cast<size_t>(f) * M, scr,
+                       scr + M);
+}
+
 // Per column tile: Doppler-tracked image column FFT, energy E_col, entropy H_col and the
 // column maxima of |X| (phase_adjustment.cu:175-246).
 template <int M>
@@ -54,15 +64,13 @@ __global__ __launch_bounds__(256) void k_pa_colfft(Job J) {
     constexpr int NT = C::NT, CT = C::CT, TPF = C::TPF, E = C::E;
     HIP_DYNAMIC_SHARED(float2, sm);
     float2* phi = sm;                                    // M
-    double* scr = reinterpret_cast<double*>(phi + M);    // M (scan) + NT (scan partials)
-    double* red = scr + M;
-    float2* work = reinterpret_cast<float2*>(red + NT);  // tile / transform buffers
+    float2* work = phi + M;                              // tile / transform buffers
     const int tile = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
     const int W = J.W;
     const int g = tid / TPF, lane = tid % TPF;
     const int n0 = tile * CT, n = n0 + g;
 
-    doppler_phase<NT>(J.rowdop + static_cast<size_t>(f) * M, M, phi, scr, red);
+    for (int m = tid; m < M; m += NT) phi[m] = J.dopphi[static_cast<size_t>(f) * M + m];
 
     const float2* X = J.x + static_cast<size_t>(f) * M * W;
     for (int idx = tid; idx < M * CT; idx += NT) {
@@ -411,7 +419,7 @@ template <int M>
 size_t colfft_lds() {
     using C = ColCfg<M>;
     const size_t work = std::max(static_cast<size_t>(M) * (C::CT + 1), static_cast<size_t>(C::CT) * lds_padded_size(M));
-    return M * sizeof(float2) + (M + C::NT) * sizeof(double) + work * sizeof(float2);
+    return M * sizeof(float2) + work * sizeof(float2);
 }
 template <int M>
 size_t autofocus_lds() {
@@ -460,6 +468,7 @@ hipError_t configure_pa_kernels() {
 }
 
 void launch_pa_colfft(const Job& J, hipStream_t s) {
+    ISAR_PULSE_SWITCH(J.M, ISAR_LAUNCH(k_pa_doppler<MM>, dim3(J.K), dim3(256), (MM + 256) * sizeof(double), s, J));
     ISAR_PULSE_SWITCH(J.M, ISAR_LAUNCH(k_pa_colfft<MM>, dim3(J.W / ColCfg<MM>::CT, J.K), dim3(ColCfg<MM>::NT),
                                        colfft_lds<MM>(), s, J));
 }

@@ -69,6 +69,7 @@ struct Job {
     FrameDecisions* dec;          // K
     float2* x;                    // K x M x W (cut)
     float2* rowdop;               // K x M
+    float2* dopphi;               // K x M: Doppler-centroid phase of the cut (k_pa_doppler)
     float* colmax_part;           // K x ceil(M/kCutBandRows) x W
     float* colmax;                // K x W
     float* colE;                  // K x W
