@@ -77,6 +77,15 @@ int isar_synth_ifds(const isar_scene_config* cfg, int len, float* iq, double* ve
 void isar_synth_radar_header(const isar_scene_config* cfg, int64_t* band_width_hz,
                              int64_t* fc_hz, double* tp_s, int64_t* fs_hz);
 
+/* The frames isar_synth_frame makes for the seeds seeds[0 .. frames-1] (each replacing cfg's
+ * seed), generated on the GPU of the calling thread: d_echo frames x M x N complex float and
+ * d_velocity frames x M doubles, device buffers. Same scene, phase-error, walk and noise draws
+ * and FP64 sums as the CPU generator, so the samples agree with it except where rounding
+ * differences cross a float or integer rounding boundary. Synchronous; allocates and frees a few
+ * small device buffers. Returns 0 or a negative isar_status code (libisar_imaging only). */
+int isar_synth_frames_device(const isar_scene_config* cfg, int frames, const uint64_t* seeds, void* d_echo,
+                             double* d_velocity);
+
 /* Write frames to a STRETCH .wbd file in the layout ioOperation reads (common.cu:796-899,
  * 1082-1100): frames of 256-byte header + int16 I/Q pairs. Samples are rounded and
  * saturated to int16. range/velocity/azimuth/pitch are per frame (may be NULL -> 0).

@@ -131,8 +131,9 @@ def build_emu(force: bool = True, sanitize: bool = True, jobs: int = 8) -> Path:
              "-I", str(INC), "-I", str(CSRC)]
     if sanitize:
         flags += ["-fsanitize=address,undefined", "-fno-sanitize-recover=undefined"]
-    # the imaging kernels only: the matrix-core kernels of nn.hip have no host emulation
-    srcs = [p for p in _hip_sources() if p.name != "nn.hip"] + [CSRC / "host" / "pipeline.cpp",
+    # the imaging kernels only: the matrix-core kernels of nn.hip and the GPU frame generator
+    # have no host emulation
+    srcs = [p for p in _hip_sources() if p.name not in ("nn.hip", "synth_gpu.hip")] + [CSRC / "host" / "pipeline.cpp",
                                                                  CSRC / "emu" / "emu_driver.cpp"]
     headers = list(INC.glob("*.h")) + list(CSRC.rglob("*.hpp"))
 

@@ -1,5 +1,6 @@
 // synth.cpp — seeded synthetic STRETCH echo generator and .wbd writer (see isar_synth.h).
 #include "isar_synth.h"
+#include "synth_model.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -42,6 +43,62 @@ uint64_t stream_seed(uint64_t seed, uint64_t purpose, uint64_t index) {
 enum : uint64_t { kScene = 1, kPhase = 2, kWalk = 3, kNoise = 4 };
 
 }  // namespace
+
+void isar_synth_model(const isar_scene_config* c, isar_synth_model_t* out) {
+    const int M = c->echo_num, S = c->n_scatterers;
+    const double dr = kC / (2.0 * c->band_width);
+    const double T = M / c->prf;
+    out->sx.assign(S, 0.0);
+    out->sy.assign(S, 0.0);
+    out->sz.assign(S, 0.0);
+    out->amp.assign(S, cd(0, 0));
+    // scene
+    {
+        Rng r(stream_seed(c->seed, kScene, 0));
+        for (int i = 0; i < S; ++i) {
+            out->sx[i] = (r.uniform() - 0.5) * c->box_x;
+            out->sy[i] = (r.uniform() - 0.5) * c->box_y;
+            out->sz[i] = (r.uniform() - 0.5) * c->box_z;
+            double g1 = r.gauss(), g2 = r.gauss();   // Rayleigh magnitude (sigma = 1), uniform phase
+            out->amp[i] = cd(g1, g2);
+        }
+    }
+    // per-pulse phase error
+    out->perr.assign(M, 0.0);
+    {
+        Rng r(stream_seed(c->seed, kPhase, 0));
+        if (c->phase_error_model == ISAR_PHASE_UNIFORM) {
+            for (int m = 0; m < M; ++m) out->perr[m] = (2.0 * r.uniform() - 1.0) * kPi;
+        } else if (c->phase_error_model == ISAR_PHASE_POLY) {
+            double ck[7] = {0, 0, 0, 0, 0, 0, 0};
+            for (int k = 2; k <= 6; ++k) ck[k] = c->poly_sigma * r.gauss();
+            for (int m = 0; m < M; ++m) {
+                double tau = M > 1 ? -1.0 + 2.0 * m / (M - 1) : 0.0, acc = 0, tp = tau * tau;
+                for (int k = 2; k <= 6; ++k) { acc += ck[k] * tp; tp *= tau; }
+                out->perr[m] = acc + c->white_sigma * r.gauss();
+            }
+        }
+    }
+    // residual range walk after coarse tracking
+    out->walk.assign(M, 0.0);
+    {
+        Rng r(stream_seed(c->seed, kWalk, 0));
+        double psi = 2.0 * kPi * r.uniform();
+        for (int m = 0; m < M; ++m) {
+            double t = m / c->prf;
+            if (c->walk_cells < 0) {
+                out->walk[m] = c->v * t + 0.5 * c->a * t * t;
+            } else if (c->walk_cells > 0) {
+                double u = T > 0 ? t / T : 0;
+                out->walk[m] = dr * c->walk_cells * (0.6 * (2 * u - 1) + 0.35 * std::sin(2 * kPi * 1.3 * u + psi)) +
+                               dr * 0.3 * r.gauss();
+            }
+        }
+    }
+}
+
+uint64_t isar_synth_noise_seed(uint64_t seed, int row) { return stream_seed(seed, kNoise, static_cast<uint64_t>(row)); }
+
 
 extern "C" {
 
@@ -117,54 +174,11 @@ int isar_synth_frame(const isar_scene_config* c, float* echo, double* velocity, 
     const int M = c->echo_num, N = c->range_num, S = c->n_scatterers;
     const double K = c->band_width / c->tp;
     const double f_hi = c->fc + c->band_width / 2;   // down-chirp starts at the top of the band
-    const double dr = kC / (2.0 * c->band_width);
     const double T = M / c->prf;
-
-    // scene
-    std::vector<double> sx(S), sy(S), sz(S);
-    std::vector<cd> amp(S);
-    {
-        Rng r(stream_seed(c->seed, kScene, 0));
-        for (int i = 0; i < S; ++i) {
-            sx[i] = (r.uniform() - 0.5) * c->box_x;
-            sy[i] = (r.uniform() - 0.5) * c->box_y;
-            sz[i] = (r.uniform() - 0.5) * c->box_z;
-            double g1 = r.gauss(), g2 = r.gauss();   // Rayleigh magnitude (sigma = 1), uniform phase
-            amp[i] = cd(g1, g2);
-        }
-    }
-    // per-pulse phase error
-    std::vector<double> perr(M, 0.0);
-    {
-        Rng r(stream_seed(c->seed, kPhase, 0));
-        if (c->phase_error_model == ISAR_PHASE_UNIFORM) {
-            for (int m = 0; m < M; ++m) perr[m] = (2.0 * r.uniform() - 1.0) * kPi;
-        } else if (c->phase_error_model == ISAR_PHASE_POLY) {
-            double ck[7] = {0, 0, 0, 0, 0, 0, 0};
-            for (int k = 2; k <= 6; ++k) ck[k] = c->poly_sigma * r.gauss();
-            for (int m = 0; m < M; ++m) {
-                double tau = M > 1 ? -1.0 + 2.0 * m / (M - 1) : 0.0, acc = 0, tp = tau * tau;
-                for (int k = 2; k <= 6; ++k) { acc += ck[k] * tp; tp *= tau; }
-                perr[m] = acc + c->white_sigma * r.gauss();
-            }
-        }
-    }
-    // residual range walk after coarse tracking
-    std::vector<double> walk(M, 0.0);
-    {
-        Rng r(stream_seed(c->seed, kWalk, 0));
-        double psi = 2.0 * kPi * r.uniform();
-        for (int m = 0; m < M; ++m) {
-            double t = m / c->prf;
-            if (c->walk_cells < 0) {
-                walk[m] = c->v * t + 0.5 * c->a * t * t;
-            } else if (c->walk_cells > 0) {
-                double u = T > 0 ? t / T : 0;
-                walk[m] = dr * c->walk_cells * (0.6 * (2 * u - 1) + 0.35 * std::sin(2 * kPi * 1.3 * u + psi)) +
-                          dr * 0.3 * r.gauss();
-            }
-        }
-    }
+    isar_synth_model_t md;
+    isar_synth_model(c, &md);
+    const std::vector<double>&sx = md.sx, &sy = md.sy, &sz = md.sz, &perr = md.perr, &walk = md.walk;
+    const std::vector<cd>& amp = md.amp;
     const double cel = std::cos(c->elevation), sel = std::sin(c->elevation);
     std::vector<double> pw(M, 0.0);
 

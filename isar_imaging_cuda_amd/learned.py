@@ -282,8 +282,8 @@ class WarmStartImager:
 # ------------------------------------------------------------------------------------------
 # data, training and evaluation
 # ------------------------------------------------------------------------------------------
-def synth_frames(config: int, seeds: list[int], threads: int = 16) -> tuple[np.ndarray, np.ndarray, dict]:
-    cfgs = [synth.preset(config, seed=s) for s in seeds]
+def synth_frames(config: int, seeds: list[int], threads: int = 16, **over) -> tuple[np.ndarray, np.ndarray, dict]:
+    cfgs = [synth.preset(config, seed=s, **over) for s in seeds]
     with cf.ThreadPoolExecutor(max_workers=threads) as ex:
         data = list(ex.map(synth.frame, cfgs))
     echo = np.stack([d[0] for d in data])
@@ -291,14 +291,38 @@ def synth_frames(config: int, seeds: list[int], threads: int = 16) -> tuple[np.n
     return echo, vel, synth.radar_header(cfgs[0])
 
 
+def synth_frames_device(config: int, seeds: list[int], **over):
+    """The same frames as synth_frames, generated on the GPU: (echo (K, M, N, 2) float32 and
+    velocity (K, M) float64 device tensors, radar header)."""
+    import torch
+    cfg = synth.preset(config, **over)
+    dev = torch.device("cuda:0")
+    e = torch.empty((len(seeds), cfg.echo_num, cfg.range_num, 2), dtype=torch.float32, device=dev)
+    v = torch.empty((len(seeds), cfg.echo_num), dtype=torch.float64, device=dev)
+    synth.frames_device(cfg, seeds, e.data_ptr(), v.data_ptr())
+    return e, v, synth.radar_header(cfg)
+
+
+def _frames(config: int, seeds: list[int], gpu_synth: bool, over: dict):
+    """(echo, velocity) device tensors and the radar header of the frames of `seeds`."""
+    import torch
+    if gpu_synth:
+        return synth_frames_device(config, seeds, **over)
+    echo, vel, h = synth_frames(config, seeds, **over)
+    dev = torch.device("cuda:0")
+    return torch.from_numpy(echo.view(np.float32)).to(dev), torch.from_numpy(vel).to(dev), h
+
+
 def _radar(h: dict, M: int, N: int) -> RadarParams:
     return RadarParams(M, N, h["band_width"], h["fc"], h["Tp"], h["Fs"])
 
 
-def make_dataset(seeds: list[int], chunk: int = 256, config: int = 4, log=print, front_iters: int = FRONT_ITERS):
+def make_dataset(seeds: list[int], chunk: int = 256, config: int = 4, log=print, front_iters: int = FRONT_ITERS,
+                 gpu_synth: bool = False, **over):
     """Features (the phase changes of the first FRONT_ITERS plain iterations) and targets
     (phi120 conj(phi_F), the remaining change of the plain iteration) for the frames of `seeds`
-    (synthesised here, imaged on the GPU)."""
+    (synthesised on the host or, with gpu_synth, on the GPU; imaged on the GPU). `over` replaces
+    fields of the synthetic preset (e.g. range_num)."""
     import torch
     dev = torch.device("cuda:0")
     feats, targets = [], []
@@ -306,13 +330,11 @@ def make_dataset(seeds: list[int], chunk: int = 256, config: int = 4, log=print,
     t0 = time.time()
     for c0 in range(0, len(seeds), chunk):
         part = seeds[c0:c0 + chunk]
-        echo, vel, h = synth_frames(config, part)
-        K, M, N = echo.shape
+        e, v, h = _frames(config, part, gpu_synth, over)
+        K, M, N = e.shape[0], e.shape[1], e.shape[2]
         if ws is None:
             ws = WarmStartImager(_radar(h, M, N), chunk, front_iters=front_iters)
             phi_end = torch.zeros((chunk, M, 2), dtype=torch.float32, device=dev)
-        e = torch.from_numpy(echo.view(np.float32)).to(dev)
-        v = torch.from_numpy(vel).to(dev)
         s = torch.cuda.current_stream(dev).cuda_stream
         ws.front(K, e.data_ptr(), v.data_ptr(), s)
         ws.features(K, s)
@@ -369,7 +391,7 @@ def iterations_to_reach(ent: np.ndarray, target: np.ndarray) -> np.ndarray:
 
 
 def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int = 4, max_iter: int = 120,
-             log=print, front_iters: int = FRONT_ITERS) -> dict:
+             log=print, front_iters: int = FRONT_ITERS, gpu_synth: bool = False, **over) -> dict:
     """Iterations until the image entropy is within 1e-5 relative of the plain 120-iteration
     result, plain (from phi0) versus learned (FRONT_ITERS iterations, the network, the entropy
     judgement, then the iteration from the judged start), on the frames of `seeds`. The learned
@@ -382,14 +404,12 @@ def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int
     ws = None
     for c0 in range(0, len(seeds), chunk):
         part = seeds[c0:c0 + chunk]
-        echo, vel, h = synth_frames(config, part)
-        K, M, N = echo.shape
+        e, v, h = _frames(config, part, gpu_synth, over)
+        K, M, N = e.shape[0], e.shape[1], e.shape[2]
         if ws is None:
             ws = WarmStartImager(_radar(h, M, N), chunk, params, front_iters=front_iters)
             img = torch.zeros((chunk, M, ws.W), dtype=torch.float32, device=dev)
             ent = torch.zeros(chunk, dtype=torch.float64, device=dev)
-        e = torch.from_numpy(echo.view(np.float32)).to(dev)
-        v = torch.from_numpy(vel).to(dev)
         s = torch.cuda.current_stream(dev).cuda_stream
         ws.front(K, e.data_ptr(), v.data_ptr(), s)
         ws.predict(K, s)
@@ -429,7 +449,7 @@ def evaluate(params: np.ndarray, seeds: list[int], chunk: int = 256, config: int
 
 
 def stage_times(params: np.ndarray, seeds: list[int], config: int = 4, front_iters: int = FRONT_ITERS,
-                repeats: int = 5) -> dict:
+                repeats: int = 5, gpu_synth: bool = False, **over) -> dict:
     """Device time per frame (ms, HIP events on the run stream, one batch of len(seeds) frames, the
     median of `repeats`) of the pieces the two chains are made of: one autofocus iteration, the
     image formation after the autofocus (MTRC, azimuth FFT), the network (features, forward,
@@ -438,12 +458,10 @@ def stage_times(params: np.ndarray, seeds: list[int], config: int = 4, front_ite
     chains and left out."""
     import torch
     dev = torch.device("cuda:0")
-    echo, vel, h = synth_frames(config, seeds)
-    K, M, N = echo.shape
+    e, v, h = _frames(config, seeds, gpu_synth, over)
+    K, M, N = e.shape[0], e.shape[1], e.shape[2]
     ws = WarmStartImager(_radar(h, M, N), K, params, front_iters=front_iters)
     img = torch.zeros((K, M, ws.W), dtype=torch.float32, device=dev)
-    e = torch.from_numpy(echo.view(np.float32)).to(dev)
-    v = torch.from_numpy(vel).to(dev)
     stream = torch.cuda.current_stream(dev)
     s = stream.cuda_stream
     ws.front(K, e.data_ptr(), v.data_ptr(), s)
@@ -486,28 +504,39 @@ def main(argv: list[str] | None = None) -> int:
     t.add_argument("--out", default=str(WEIGHTS_DIR / "warm_start_m256.npz"))
     t.add_argument("--eval-frames", type=int, default=256)
     t.add_argument("--front-iters", type=int, default=FRONT_ITERS)
+    t.add_argument("--config", type=int, default=4, help="synthetic preset of the training frames")
+    t.add_argument("--range-num", type=int, default=0, help="range cells of the training frames (0: the preset's)")
+    t.add_argument("--gpu-synth", action="store_true", help="synthesise the frames on the GPU")
+    t.add_argument("--chunk", type=int, default=256)
     ev = sub.add_parser("eval")
     ev.add_argument("--weights", default=None)
     ev.add_argument("--frames", type=int, default=1000)
     ev.add_argument("--seed0", type=int, default=10_000_001)
+    ev.add_argument("--config", type=int, default=4)
+    ev.add_argument("--range-num", type=int, default=0)
+    ev.add_argument("--gpu-synth", action="store_true")
+    ev.add_argument("--chunk", type=int, default=256)
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
+    over = {"range_num": a.range_num} if a.range_num else {}
     if a.cmd == "train":
         t0 = time.time()
-        feats, targets = make_dataset(list(range(a.seed0, a.seed0 + a.frames)), log=log, front_iters=a.front_iters)
+        feats, targets = make_dataset(list(range(a.seed0, a.seed0 + a.frames)), chunk=a.chunk, config=a.config, log=log,
+                                      front_iters=a.front_iters, gpu_synth=a.gpu_synth, **over)
         params, hist = train(feats, targets, epochs=a.epochs, log=log)
         out = Path(a.out)
         out.parent.mkdir(parents=True, exist_ok=True)
         np.savez(out, params=params, loss=np.asarray(hist), frames=a.frames, epochs=a.epochs,
-                 front_iters=a.front_iters)
+                 front_iters=a.front_iters, config=a.config, range_num=a.range_num)
         log(f"[train] saved {out} after {time.time() - t0:.1f} s")
         if a.eval_frames:
-            res = evaluate(params, list(range(10_000_001, 10_000_001 + a.eval_frames)), log=log,
-                           front_iters=a.front_iters)
+            res = evaluate(params, list(range(10_000_001, 10_000_001 + a.eval_frames)), chunk=min(a.chunk, a.eval_frames),
+                           config=a.config, log=log, front_iters=a.front_iters, gpu_synth=a.gpu_synth, **over)
             log(json.dumps(res))
     else:
         params, fi = load_weights(a.weights)
-        res = evaluate(params, list(range(a.seed0, a.seed0 + a.frames)), log=log, front_iters=fi)
+        res = evaluate(params, list(range(a.seed0, a.seed0 + a.frames)), chunk=min(a.chunk, a.frames), config=a.config,
+                       log=log, front_iters=fi, gpu_synth=a.gpu_synth, **over)
         log(json.dumps(res))
     return 0
 

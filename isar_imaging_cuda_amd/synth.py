@@ -106,3 +106,16 @@ def write_wbd(path: str | Path, echo: np.ndarray, cfg: SceneConfig, range_m=None
                               h["Tp"], h["Fs"], *[arr(a) for a in keep])
     if rc != 0:
         raise OSError(f"cannot write {path}")
+
+
+def frames_device(cfg: SceneConfig, seeds, d_echo: int, d_velocity: int) -> None:
+    """The frames `frame` makes for each seed of `seeds`, generated on the GPU into device buffers
+    (d_echo: len(seeds) x M x N complex64, d_velocity: len(seeds) x M float64); isar_synth.h's
+    isar_synth_frames_device in libisar_imaging.so."""
+    from . import _check
+    from . import lib as imaging_lib
+    L = imaging_lib()
+    L.isar_synth_frames_device.argtypes = [C.POINTER(SceneConfig), C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    s = np.ascontiguousarray(np.asarray(list(seeds), dtype=np.uint64))
+    _check(L.isar_synth_frames_device(C.byref(cfg), int(s.size), s.ctypes.data_as(C.c_void_p), C.c_void_p(d_echo),
+                                      C.c_void_p(d_velocity)), "isar_synth_frames_device")

@@ -308,3 +308,37 @@ def test_final_judge_falls_back_to_the_plain_image():
             assert torch.equal(out[k], plain[k]) and el > ep * (1 + L.ENTROPY_REL) - 1e-12, k
     print(f"{rejected} of {K} frames fell back to the plain image")
     assert rejected > 0
+
+
+@pytest.mark.gpu
+def test_1024_pulse_network_never_worse_than_plain():
+    """The shipped 1024-pulse network (trained on 1024 x 2048 frames of the 1024 x 16384 scene:
+    1000 scatterers, 7 km/s) through the product path with the final judgement: on held-out
+    frames of that scene every returned image has an entropy no worse than the plain
+    120-iteration image's (within 1e-5 relative)."""
+    torch = _torch()
+    import isar_imaging_cuda_amd as isar
+    from isar_imaging_cuda_amd import learned as L
+    from oracle import oracle as O
+    try:
+        params, front_iters = L.load_weights(M=1024)
+    except Exception as e:
+        pytest.skip(str(e))
+    K = 8
+    e, v, h = L.synth_frames_device(5, list(range(50_000_001, 50_000_001 + K)), range_num=2048)
+    M, N = e.shape[1], e.shape[2]
+    print(f"[case] 1024-pulse warm start, {K} frames {M}x{N}", flush=True)
+    params_r = isar.RadarParams(M, N, h["band_width"], h["fc"], h["Tp"], h["Fs"])
+    s = torch.cuda.current_stream().cuda_stream
+    ws = L.WarmStartImager(params_r, K, params, front_iters=front_iters)
+    out = torch.zeros((K, M, 512), dtype=torch.float32, device=e.device)
+    acc = ws.run(K, e.data_ptr(), v.data_ptr(), out.data_ptr(), 120 - front_iters, s).cpu().numpy()
+    plain = torch.zeros_like(out)
+    ws.imager.run_device(e.data_ptr(), v.data_ptr(), plain.data_ptr(), K, isar.Options(), stream=s)
+    torch.cuda.synchronize()
+    for k in range(K):
+        eo, ep = O.image_entropy(out[k].cpu().numpy()), O.image_entropy(plain[k].cpu().numpy())
+        assert eo <= ep * (1 + L.ENTROPY_REL), (k, eo, ep)
+        if not acc[k]:
+            assert torch.equal(out[k], plain[k])
+    print(f"learned image kept on {int(acc.sum())} of {K} frames")
