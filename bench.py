@@ -387,7 +387,7 @@ def main() -> int:
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.large_only:
-        print(json.dumps({"large_batch": large_batch()}))
+        print(json.dumps({"large_batch": large_batch(learned_frames=0 if args.no_learned else 16)}))
         return 0
     if args.subapertures_only:
         print(json.dumps({"subapertures": subapertures(autotune=not args.sa_fixed)}))
