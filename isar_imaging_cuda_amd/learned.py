@@ -303,14 +303,28 @@ def synth_frames_device(config: int, seeds: list[int], **over):
     return e, v, synth.radar_header(cfg)
 
 
+def host_echo(echo: np.ndarray) -> np.ndarray:
+    """complex64 frames (K, M, N) as the float32 (K, M, N, 2) array of (re, im) pairs that the
+    device echo tensors hold (a plain float32 view would be (K, M, 2N))."""
+    if echo.ndim != 3 or echo.dtype != np.complex64:
+        raise ImagingError(f"host_echo: expected complex64 (K, M, N) frames, got {echo.dtype} {echo.shape}")
+    K, M, N = echo.shape
+    return np.ascontiguousarray(echo).view(np.float32).reshape(K, M, N, 2)
+
+
 def _frames(config: int, seeds: list[int], gpu_synth: bool, over: dict):
-    """(echo, velocity) device tensors and the radar header of the frames of `seeds`."""
+    """(echo (K, M, N, 2) float32, velocity (K, M) float64) device tensors and the radar header
+    of the frames of `seeds`; the callers take K, M, N from the echo tensor's shape."""
     import torch
     if gpu_synth:
-        return synth_frames_device(config, seeds, **over)
-    echo, vel, h = synth_frames(config, seeds, **over)
-    dev = torch.device("cuda:0")
-    return torch.from_numpy(echo.view(np.float32)).to(dev), torch.from_numpy(vel).to(dev), h
+        e, v, h = synth_frames_device(config, seeds, **over)
+    else:
+        echo, vel, h = synth_frames(config, seeds, **over)
+        dev = torch.device("cuda:0")
+        e, v = torch.from_numpy(host_echo(echo)).to(dev), torch.from_numpy(vel).to(dev)
+    if e.dim() != 4 or e.shape[3] != 2 or tuple(v.shape) != tuple(e.shape[:2]):
+        raise ImagingError(f"frame tensors of shape {tuple(e.shape)} and {tuple(v.shape)}; expected (K, M, N, 2) and (K, M)")
+    return e, v, h
 
 
 def _radar(h: dict, M: int, N: int) -> RadarParams:

@@ -60,6 +60,21 @@ def test_oracle_relu_network_is_piecewise_linear():
     assert np.allclose(b, 2 * a)
 
 
+def test_host_frames_keep_their_range_cells():
+    """Host-synthesised frames go to the device as (K, M, N, 2) float32, the layout of the GPU
+    generator's tensors, so that K, M, N read from the tensor are the frames' own: a plain float32
+    view of complex64 (K, M, N) frames is (K, M, 2N), which sized an imaging context for twice the
+    range cells of its echo buffer."""
+    from isar_imaging_cuda_amd import ImagingError
+    from isar_imaging_cuda_amd import learned as L
+    echo = (np.arange(2 * 3 * 4) + 1j * -np.arange(2 * 3 * 4)).astype(np.complex64).reshape(2, 3, 4)
+    h = L.host_echo(echo)
+    assert h.shape == (2, 3, 4, 2) and h.dtype == np.float32
+    assert np.array_equal(h[..., 0], echo.real) and np.array_equal(h[..., 1], echo.imag)
+    with pytest.raises(ImagingError):
+        L.host_echo(echo.astype(np.complex128))
+
+
 # ------------------------------------------------------------------------------------------
 # GPU
 # ------------------------------------------------------------------------------------------
