@@ -369,6 +369,14 @@ __device__ __forceinline__ float2 opaque(float2 v) {
     return v;
 }
 
+// Column tile of workgroup b among the `tiles` tiles (gridDim.x) of a frame, ordered so that one
+// XCD runs adjacent tiles. Workgroups go to the 8 XCDs round-robin by linear id (b + tiles f), so
+// with tiles % 8 == 0 XCD x takes tiles [x tiles/8, (x+1) tiles/8) of every frame: tiles narrower
+// than a 128-byte line then share its fetch in one L2 instead of each XCD fetching it.
+__device__ __forceinline__ int xcd_tile(int b, int tiles) {
+    return (tiles & 7) ? b : (b & 7) * (tiles >> 3) + (b >> 3);
+}
+
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int lds_padded_size(int n) { return n + (n >> 4) + 1; }
 

@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void k_mtrc_czt(Job J) {
     using C = CztCfg<F>;
     constexpr int CT = C::CT, TPF = C::TPF, E = C::E, NT = C::NT;
     HIP_DYNAMIC_SHARED(float2, sm);
-    const int tile = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x), f = blockIdx.y, tid = threadIdx.x;
     const int M = J.M, W = J.W;
     const int g = tid / TPF, lane = tid % TPF;
     const int n0 = tile * CT, n = n0 + g;
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(256) void k_col_window(Job J) {
     using C = ColCfg<M>;
     constexpr int CT = C::CT, TPF = C::TPF, E = C::E, NT = C::NT;
     HIP_DYNAMIC_SHARED(float2, sm);
-    const int tile = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x), f = blockIdx.y, tid = threadIdx.x;
     const int W = J.W;
     const int g = tid / TPF, lane = tid % TPF;
     const int n0 = tile * CT;
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(256) void k_post_nomtrc(Job J) {
     using C = ColCfg<M>;
     constexpr int CT = C::CT, TPF = C::TPF, E = C::E, NT = C::NT;
     HIP_DYNAMIC_SHARED(float2, sm);
-    const int tile = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x), f = blockIdx.y, tid = threadIdx.x;
     const int W = J.W;
     const int g = tid / TPF, lane = tid % TPF;
     const int n0 = tile * CT, n = n0 + g;

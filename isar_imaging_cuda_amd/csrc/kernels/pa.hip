@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void k_pa_colfft(Job J) {
     HIP_DYNAMIC_SHARED(float2, sm);
     float2* phi = sm;                                    // M
     float2* work = phi + M;                              // tile / transform buffers
-    const int tile = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x), f = blockIdx.y, tid = threadIdx.x;
     const int W = J.W;
     const int g = tid / TPF, lane = tid % TPF;
     const int n0 = tile * CT, n = n0 + g;

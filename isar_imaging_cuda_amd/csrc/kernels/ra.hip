@@ -28,27 +28,58 @@ __device__ __forceinline__ int out_pos(int tid, int e) {
 // the first-pass layout (range_alignment.cu:13-18, high_speed_compensation.cu:4-54). With
 // if_hrrp the STRETCH rows carry the window twice (getHRRP windows them in place first,
 // common.cu:563-566) unless ONE_WINDOW asks for the rows as getHRRP transforms them.
+//
+// The HPC phase exp(j pi vk n^2) at the first-pass positions n = tid + e S, S = N / 16, is
+// factorised as exp(j pi vk tid^2) * d^e * exp(j pi vk S^2 e^2) with d = exp(j pi 2 vk S tid):
+// three FP64-reduced evaluations per thread (tid^2, d, d^4) and one per lane e < 16 of each wave
+// (the e^2 factor, read by the other lanes), instead of sixteen. exp(j pi vk tid^2) d^e is walked
+// along each group of four elements by one product with d, and the group base steps by d^4, so
+// no factor is more than six products from an FP64-reduced value and only the running factor,
+// the group base, d, d^4 and the lane's e^2 factor stay live beside the row. Every FP64 argument
+// is a part of vk n^2 and is reduced at least as exactly as vk n^2 itself.
 template <int N, bool ONE_WINDOW = false>
 __device__ __forceinline__ void load_row_raw(float2 (&x)[kE], const Job& J, int f, int m, int tid_) {
+    static_assert(FftPlan<N, kE>::R[0] == 16, "factorised phase assumes a radix-16 first pass");
     const int tid = opaque(tid_);
     const long long erow = echo_first_row(J, f) + m;
     const float2* row = J.data_type == 1 ? J.pre + (static_cast<size_t>(f) * J.M + m) * N
                                          : J.echo + static_cast<size_t>(erow) * N;
     const bool hpc = J.if_hpc && J.data_type == 2;
-    const double vk = hpc ? J.hpc_k * J.velocity[erow] : 0.0;
     const bool hrrp_window = !ONE_WINDOW && J.if_hrrp && J.data_type == 2;
-#pragma unroll
-    for (int e = 0; e < kE; ++e) {
-        const int n = in_pos<N>(tid, e);
-        float2 v = row[n];
-        if (hpc) {
-            const double dn = static_cast<double>(n);
-            v = cmul(v, expjpi_reduced(vk * dn * dn));
-        }
+    auto window = [&](int n) {
         float w = J.win_range[n];
         if (hrrp_window) w *= w;
-        if (n & 1) w = -w;
-        x[e] = cscale(v, w);
+        return (n & 1) ? -w : w;
+    };
+    if (!hpc) {
+#pragma unroll
+        for (int e = 0; e < kE; ++e) {
+            const int n = in_pos<N>(tid, e);
+            x[e] = cscale(row[n], window(n));
+        }
+        return;
+    }
+    constexpr double S = N / kE;
+    const double vk = J.hpc_k * J.velocity[erow];
+    const double t = static_cast<double>(tid);
+    const double lin = 2.0 * S * vk * t;
+    const float2 d1 = expjpi_reduced(lin);
+    const float2 d4 = expjpi_reduced(4.0 * lin);
+    float2 base = expjpi_reduced(vk * t * t);
+    const double e0 = static_cast<double>(tid & 15);
+    const float2 c = expjpi_reduced((S * S) * vk * (e0 * e0));
+#pragma unroll
+    for (int q = 0; q < kE / 4; ++q) {
+        float2 h = base;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e = 4 * q + r;
+            const int n = in_pos<N>(tid, e);
+            const float2 ce = make_float2(wave_read_lane(c.x, e), wave_read_lane(c.y, e));
+            x[e] = cscale(cmul(row[n], cmul(h, ce)), window(n));
+            if (r < 3) h = cmul(h, d1);
+        }
+        if (q + 1 < kE / 4) base = cmul(base, d4);
     }
 }
 
