@@ -227,6 +227,7 @@ def subapertures(count: int = 64, replays: int = 10, autotune: bool = True) -> d
     e1.record(s)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / replays
+    tuning = plan.tuning()
     plan.close()
     # minimum-pass bytes of every window of the launch; U = 64 selected bins per window (the 8MU
     # term is at most 0.13 % of a window's bytes at 512 x 4096, so the assumption barely matters)
@@ -236,6 +237,8 @@ def subapertures(count: int = 64, replays: int = 10, autotune: bool = True) -> d
             "ms_per_graph_launch": round(ms, 4), "windows_per_s": round(count / (ms / 1e3), 1),
             "pulses_per_s": round(float(pulses.sum()) / (ms / 1e3), 1),
             "ra_chunk_rows": {int(m): int(c) for m, c in zip(plan.group_pulses, chosen)},
+            "tuning": {m: f"chunk {t['ra_chunk']}, autofocus form {t['autofocus_mode']}, chirp-z {t['czt_length']} x "
+                          f"{t['czt_threads']} threads" for m, t in tuning.items()},
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                          "algorithmic_bytes_per_launch": alg, "selected_bins_assumed": 64}}

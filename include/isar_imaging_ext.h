@@ -204,9 +204,25 @@ ISAR_API int isar_get_ra_chunk(isar_context* ctx);
  * be exercised on small frames. */
 ISAR_API int isar_set_ra_split(isar_context* ctx, int mode);
 ISAR_API int isar_get_ra_split(isar_context* ctx);
+/* Chirp-z transform length F of the MTRC's Bluestein convolution (mtrc.cu:125-192). The default
+ * (0) is the reference's nextPow2(2M - 1); for 192 and 384 pulses the shorter 3 * 2^k lengths 384
+ * and 768 (mixed-radix transforms) are accepted too. Every length gives the same transform up to
+ * FP32 rounding. isar_czt_lengths lists the accepted lengths, the default first, and returns how
+ * many (at most max). Setting a new length waits for the device, then re-forms the chirp-z
+ * tables. */
+ISAR_API int isar_set_czt_length(isar_context* ctx, int length);
+ISAR_API int isar_get_czt_length(isar_context* ctx);
+ISAR_API int isar_czt_lengths(isar_context* ctx, int32_t* out, int max);
+/* Chirp-z column tile: 256 (default) or 512 threads per workgroup, i.e. 256 / (F / E) or twice as
+ * many range bins per workgroup. Results are identical bit for bit. */
+ISAR_API int isar_set_czt_threads(isar_context* ctx, int threads);
+ISAR_API int isar_get_czt_threads(isar_context* ctx);
 /* Autotuner: times every range-alignment chunk size on these frames (`repeats` runs each after
  * one warm-up, on `stream`), keeps the fastest for the context and returns it in *chosen; then
- * times the autofocus forms with that chunk size and keeps the fastest (isar_get_autofocus_mode).
+ * times the autofocus forms with that chunk size and keeps the fastest (isar_get_autofocus_mode);
+ * then, for runs with MTRC, times every chirp-z length and column tile on the chirp-z launches
+ * alone (through the kernel probe, which it leaves enabled for the stage it found, with its
+ * accumulated time cleared) and keeps the fastest (isar_get_czt_length, isar_get_czt_threads).
  * Synchronous; the image buffer receives the last run's images. */
 ISAR_API int isar_autotune(isar_context* ctx, int frames, const void* d_echo, const double* d_velocity, float* d_img,
                            const isar_options* opt, int repeats, int32_t* chosen, void* stream);

@@ -37,12 +37,17 @@ ISAR_API int isar_sa_plan_image_offsets(const isar_sa_plan* plan, int64_t* offse
  * pointers and options and replayed afterwards. Asynchronous on `stream`. */
 ISAR_API int isar_sa_plan_run(isar_sa_plan* plan, const void* d_dwell, const double* d_velocity, float* d_img,
                               const isar_options* opt, int use_graph, void* stream);
-/* Times every range-alignment chunk size per pulse-count group (`repeats` runs each after one
- * warm-up) and keeps the fastest; chosen[g] receives the choice of group g (groups in
- * increasing pulse count; pass NULL to skip). Synchronous. Invalidates a captured graph. */
+/* Tunes every pulse-count group like isar_autotune (range-alignment chunk rows, autofocus form,
+ * chirp-z length and column tile; `repeats` runs each after one warm-up), then times whole graph
+ * replays for a few combinations of the groups' autofocus forms and keeps the fastest; chosen[g]
+ * receives the chunk rows of group g (groups in increasing pulse count; pass NULL to skip; the
+ * rest is read with isar_sa_plan_group_tuning). Synchronous. Invalidates a captured graph. */
 ISAR_API int isar_sa_plan_autotune(isar_sa_plan* plan, const void* d_dwell, const double* d_velocity,
                                    float* d_img, const isar_options* opt, int repeats, int32_t* chosen);
 ISAR_API int isar_sa_plan_groups(const isar_sa_plan* plan, int32_t* pulses, int32_t* windows);
+/* Tuning of group g (groups in increasing pulse count): tuning[0] range-alignment chunk rows,
+ * [1] autofocus form (isar_autofocus_mode), [2] chirp-z length, [3] chirp-z workgroup threads. */
+ISAR_API int isar_sa_plan_group_tuning(const isar_sa_plan* plan, int group, int32_t* tuning);
 
 #ifdef __cplusplus
 }

@@ -267,6 +267,12 @@ class Imager:
         L.isar_set_autofocus_mode.argtypes = [C.c_void_p, C.c_int]
         _check(L.isar_set_autofocus_mode(self._h, int(mode)), "isar_set_autofocus_mode")
 
+    @property
+    def autofocus_mode(self) -> int:
+        L = lib()
+        L.isar_get_autofocus_mode.argtypes = [C.c_void_p]
+        return int(L.isar_get_autofocus_mode(self._h))
+
     def set_graph_mode(self, on: bool) -> None:
         """Replay a captured HIP graph for full runs whose arguments repeat."""
         L = lib()
@@ -293,9 +299,46 @@ class Imager:
         L.isar_set_ra_split.argtypes = [C.c_void_p, C.c_int]
         _check(L.isar_set_ra_split(self._h, int(mode)), "isar_set_ra_split")
 
+    def czt_lengths(self) -> list[int]:
+        """Chirp-z transform lengths this context accepts, the default (nextPow2(2M - 1)) first."""
+        L = lib()
+        L.isar_czt_lengths.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int]
+        out = (C.c_int32 * 4)()
+        n = L.isar_czt_lengths(self._h, out, 4)
+        if n < 1:
+            _check(n, "isar_czt_lengths")
+        return [int(out[i]) for i in range(n)]
+
+    def set_czt_length(self, length: int) -> None:
+        """Chirp-z transform length (one of czt_lengths(); 0 = the default). Waits for the device
+        and re-forms the chirp-z tables; the transform is the same up to FP32 rounding."""
+        L = lib()
+        L.isar_set_czt_length.argtypes = [C.c_void_p, C.c_int]
+        _check(L.isar_set_czt_length(self._h, int(length)), "isar_set_czt_length")
+
+    @property
+    def czt_length(self) -> int:
+        L = lib()
+        L.isar_get_czt_length.argtypes = [C.c_void_p]
+        return int(L.isar_get_czt_length(self._h))
+
+    def set_czt_threads(self, threads: int) -> None:
+        """Chirp-z workgroup width, 256 or 512 threads (bit-identical results)."""
+        L = lib()
+        L.isar_set_czt_threads.argtypes = [C.c_void_p, C.c_int]
+        _check(L.isar_set_czt_threads(self._h, int(threads)), "isar_set_czt_threads")
+
+    @property
+    def czt_threads(self) -> int:
+        L = lib()
+        L.isar_get_czt_threads.argtypes = [C.c_void_p]
+        return int(L.isar_get_czt_threads(self._h))
+
     def autotune(self, echo_ptr: int, velocity_ptr: int | None, img_ptr: int, frames: int,
                  options: Options | None = None, repeats: int = 3, stream: int | None = None) -> int:
-        """Time every range-alignment chunk size on these frames, keep the fastest and return it."""
+        """Time every range-alignment chunk size on these frames, keep the fastest and return it;
+        then the autofocus forms, and with MTRC the chirp-z lengths and column tiles (czt_length,
+        czt_threads)."""
         L = lib()
         L.isar_autotune.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                     C.c_void_p, C.c_void_p]

@@ -1,7 +1,7 @@
 // emu_driver.cpp — runs the kernel sources on the CPU (hip_emu.hpp) through the same C ABI as
 // the device library. Test infrastructure: built with AddressSanitizer by the CPU test suite.
 //
-// usage: isar_emu_driver <input.bin> <output.bin> [autofocus_mode [ra_split [static]]]
+// usage: isar_emu_driver <input.bin> <output.bin> [autofocus_mode [ra_split [static [czt_length [czt_threads]]]]]
 // static = 1: also images the batch, then its first frame alone, through isar_run_device_static
 // and requires both to equal the context's images bit for bit (exit 10-12 otherwise).
 // input : int32 M, N, W, K; int64 band_width, fc; double Tp; int64 Fs; isar_options;
@@ -16,8 +16,9 @@
 #include "isar_imaging_ext.h"
 
 int main(int argc, char** argv) {
-    if (argc < 3 || argc > 6) {
-        std::fprintf(stderr, "usage: %s input.bin output.bin [autofocus_mode [ra_split [static]]]\n", argv[0]);
+    if (argc < 3 || argc > 8) {
+        std::fprintf(stderr, "usage: %s input.bin output.bin [autofocus_mode [ra_split [static [czt_length [czt_threads]]]]]\n",
+                     argv[0]);
         return 2;
     }
     FILE* in = std::fopen(argv[1], "rb");
@@ -43,7 +44,7 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "create: %s\n", isar_last_error());
         return 6;
     }
-    if (argc == 5 && isar_set_ra_split(ctx, std::atoi(argv[4])) != 0) {
+    if (argc >= 5 && isar_set_ra_split(ctx, std::atoi(argv[4])) != 0) {
         std::fprintf(stderr, "ra split: %s\n", isar_last_error());
         return 6;
     }
@@ -51,12 +52,20 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "autofocus mode: %s\n", isar_last_error());
         return 6;
     }
+    if (argc >= 7 && isar_set_czt_length(ctx, std::atoi(argv[6])) != 0) {
+        std::fprintf(stderr, "chirp-z length: %s\n", isar_last_error());
+        return 6;
+    }
+    if (argc >= 8 && isar_set_czt_threads(ctx, std::atoi(argv[7])) != 0) {
+        std::fprintf(stderr, "chirp-z threads: %s\n", isar_last_error());
+        return 6;
+    }
     std::vector<float> img(static_cast<size_t>(K) * M * W);
     if (isar_run_host(ctx, K, echo.data(), vel.data(), opt.warm_start ? phi0.data() : nullptr, img.data(), &opt) != 0) {
         std::fprintf(stderr, "run: %s\n", isar_last_error());
         return 7;
     }
-    if (argc == 6 && std::atoi(argv[5]) == 1) {
+    if (argc >= 6 && std::atoi(argv[5]) == 1) {
         // the emulated device memory is host memory: the host arrays serve as device buffers
         std::vector<float> img_s(img.size());
         const void* ph = opt.warm_start ? phi0.data() : nullptr;

@@ -210,7 +210,7 @@ int Context::init_in(const isar_radar_params& params, int frames, void* base, hi
     J.hpc_k = -4.0 * (B / p.Tp) / kLightSpeed / (Fs * Fs);
     J.czt_c0 = B / (2.0 * fc);
     J.czt_posa = B / (fc * (static_cast<double>(J.W) - 1.0));
-    czt_table = t.czt_table;
+    tables = t;
     J.czt_table = t.czt_table;
     J.czt_pre = t.czt_pre;
     J.czt_post = t.czt_post;
@@ -221,6 +221,39 @@ int Context::init_in(const isar_radar_params& params, int frames, void* base, hi
     launch_czt_chirps(J.M, J.W, F, J.czt_c0, J.czt_posa, t.czt_pre, t.czt_post, s);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("workspace tables: ") + hipGetErrorString(e));
+    return ISAR_OK;
+}
+
+int czt_length_candidates(int M, int* out) {
+    const int n = 2 * M - 1;
+    const int p2 = next_pow2_ref(n);
+    int c = 0;
+    out[c++] = p2;
+    for (int f : {384, 768})   // the 3 * 2^k lengths with a transform plan
+        if (f >= n && f < p2) out[c++] = f;
+    return c;
+}
+
+int Context::set_czt_length(int length) {
+    int cand[2];
+    const int nc = czt_length_candidates(p.echo_num, cand);
+    if (length == 0) length = cand[0];
+    if (std::find(cand, cand + nc, length) == cand + nc)
+        return fail(ISAR_ERR_INVALID, "chirp-z length " + std::to_string(length) + " is not available for " +
+                                          std::to_string(p.echo_num) + " pulses");
+    if (length == F) return ISAR_OK;
+    // the tables may be in use by runs still in flight on any stream
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("synchronize: ") + hipGetErrorString(e));
+    F = length;
+    tmpl.F = F;
+    drop_graph();
+    (void)hipGetLastError();
+    launch_czt_table(tmpl.M, tmpl.W, F, tmpl.czt_c0, tmpl.czt_posa, tables.czt_table, nullptr);
+    launch_czt_chirps(tmpl.M, tmpl.W, F, tmpl.czt_c0, tmpl.czt_posa, tables.czt_pre, tables.czt_post, nullptr);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("chirp-z tables: ") + hipGetErrorString(e));
     return ISAR_OK;
 }
 
@@ -253,12 +286,14 @@ int Context::run_frames(int frames, const void* echo, const double* velocity, co
     k.ra_chunk = ra_chunk;
     k.af_mode = af_mode;
     k.ra_split = ra_split;
+    k.czt_len = F;
+    k.czt_threads = czt_threads;
     k.hrrp = hrrp_out;
     const GraphKey& g0 = graph_key;
     const bool same = graph_exec && k.frames == g0.frames && k.echo == g0.echo && k.velocity == g0.velocity &&
                       k.frame_row == g0.frame_row && k.phi0 == g0.phi0 && k.img == g0.img &&
                       k.frame_img == g0.frame_img && k.ra_chunk == g0.ra_chunk && k.af_mode == g0.af_mode && k.ra_split == g0.ra_split &&
-                      k.hrrp == g0.hrrp && std::memcmp(&k.opt, &g0.opt, sizeof(isar_options)) == 0;
+                      k.czt_len == g0.czt_len && k.czt_threads == g0.czt_threads && k.hrrp == g0.hrrp && std::memcmp(&k.opt, &g0.opt, sizeof(isar_options)) == 0;
     if (!same) {
         // capture the launches on an internal stream; the graph is then launched on the caller's
         drop_graph();
@@ -324,6 +359,7 @@ int Context::enqueue(int frames, const void* echo, const double* velocity, const
     J.frame_img = frame_img;
     J.ra_chunk = ra_chunk;
     J.ra_split = ra_split;
+    J.czt_threads = czt_threads;
     J.phi0_in = static_cast<const float2*>(phi0);
     J.img = img;
     J.hrrp = opt.if_hrrp && mode != ISAR_RUN_BACK ? hrrp_out : nullptr;
@@ -429,6 +465,44 @@ int Context::autotune(int frames, const void* echo, const double* velocity, cons
         }
         af_mode = best_mode;
         tuned_ms = (best_af > 0.0f ? best_af : best_ms) / static_cast<float>(repeats);
+    }
+    // chirp-z length and workgroup width, timed on the chirp-z launches alone (a whole run is too
+    // noisy for a stage of a few percent); then the kept configuration is timed again
+    if (rc == ISAR_OK && opt.if_mtrc && opt.stage_mode == ISAR_RUN_FULL) {
+        const int saved_probe = probe_stage;
+        int cand[2];
+        const int nc = czt_length_candidates(p.echo_num, cand);
+        double best_czt = -1.0;
+        int best_len = F, best_threads = czt_threads;
+        for (int i = 0; i < nc && rc == ISAR_OK; ++i) {
+            for (int t : {256, 512}) {
+                rc = set_czt_length(cand[i]);
+                if (rc) break;
+                czt_threads = t;
+                rc = probe_enable(ISAR_STAGE_MTRC_CZT);
+                if (rc == ISAR_OK) rc = run_frames(frames, echo, velocity, frame_row, nullptr, img, frame_img, opt, s);
+                if (rc == ISAR_OK) rc = probe_read(nullptr, nullptr);   // the warm-up run
+                for (int r = 0; r < repeats && rc == ISAR_OK; ++r)
+                    rc = run_frames(frames, echo, velocity, frame_row, nullptr, img, frame_img, opt, s);
+                double ms = 0.0;
+                int launches = 0;
+                if (rc == ISAR_OK) rc = probe_read(&ms, &launches);
+                if (rc) break;
+                if (launches > 0 && (best_czt < 0.0 || ms / launches < best_czt)) {
+                    best_czt = ms / launches;
+                    best_len = cand[i];
+                    best_threads = t;
+                }
+            }
+        }
+        const int prc = probe_enable(saved_probe);
+        if (rc == ISAR_OK) rc = prc;
+        if (rc == ISAR_OK) rc = set_czt_length(best_len);
+        czt_threads = best_threads;
+        if (rc == ISAR_OK) {
+            const float ms = time_runs();
+            if (ms >= 0.0f) tuned_ms = ms / static_cast<float>(repeats);
+        }
     }
     if (chosen) *chosen = best;
     (void)hipEventDestroy(a);
@@ -628,6 +702,30 @@ ISAR_API int isar_set_ra_split(isar_context* ctx, int mode) {
 }
 
 ISAR_API int isar_get_ra_split(isar_context* ctx) { return ctx ? ctx->ra_split : -1; }
+
+ISAR_API int isar_set_czt_length(isar_context* ctx, int length) {
+    if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");
+    return ctx->set_czt_length(length);
+}
+
+ISAR_API int isar_get_czt_length(isar_context* ctx) { return ctx ? ctx->F : 0; }
+
+ISAR_API int isar_czt_lengths(isar_context* ctx, int32_t* out, int max) {
+    if (!ctx || !out || max < 1) return isar::fail(ISAR_ERR_INVALID, "isar_czt_lengths: bad arguments");
+    int cand[2];
+    const int n = isar::czt_length_candidates(ctx->p.echo_num, cand);
+    for (int i = 0; i < std::min(n, max); ++i) out[i] = cand[i];
+    return std::min(n, max);
+}
+
+ISAR_API int isar_set_czt_threads(isar_context* ctx, int threads) {
+    if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");
+    if (threads != 256 && threads != 512) return isar::fail(ISAR_ERR_INVALID, "chirp-z workgroups have 256 or 512 threads");
+    ctx->czt_threads = threads;
+    return ISAR_OK;
+}
+
+ISAR_API int isar_get_czt_threads(isar_context* ctx) { return ctx ? ctx->czt_threads : 0; }
 
 ISAR_API int isar_set_autofocus_mode(isar_context* ctx, int mode) {
     if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");

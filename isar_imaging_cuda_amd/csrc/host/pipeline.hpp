@@ -12,13 +12,21 @@ namespace isar {
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 
+// tables shared by every frame of a context: chirp-z filter spectra and chirps, range window
+struct SharedTables {
+    float2* czt_table;
+    float2* czt_pre;
+    float2* czt_post;
+    float* win_range;
+};
+
 struct Context {
     isar_radar_params p{};
     int max_frames = 0;
     int F = 0;
     Job tmpl{};                       // workspace pointers; per-run fields filled in run()
     std::vector<void*> allocations;   // everything allocated at creation
-    float2* czt_table = nullptr;
+    SharedTables tables{};            // inside the workspace
     int last_frames = 0;
     int front_frames = 0;             // batch size of the last run that formed the cut
     // kernel probe: event pairs recorded around each launch of one kernel family
@@ -70,7 +78,7 @@ struct Context {
         float* img = nullptr;
         const long long* frame_img = nullptr;
         isar_options opt{};
-        int ra_chunk = 0, af_mode = 0, ra_split = 0;
+        int ra_chunk = 0, af_mode = 0, ra_split = 0, czt_len = 0, czt_threads = 0;
         const float2* hrrp = nullptr;
     };
     bool graph_mode = false;
@@ -91,6 +99,10 @@ struct Context {
     float mode_ms[4] = {0, 0, 0, 0};      // time of one run per autofocus form (index: isar_autofocus_mode)
     int ra_chunk = kRaChunkRowsDefault;   // tunable: rows per workgroup in the alignment update
     int ra_split = 0;                     // alignment update grids per level (isar_set_ra_split)
+    int czt_threads = 256;                // chirp-z workgroup width (isar_set_czt_threads)
+    // chirp-z transform length (one of czt_length_candidates; 0 = the default): waits for the
+    // device, then re-forms the chirp-z tables
+    int set_czt_length(int length);
     // times every range-alignment chunk size on these frames and keeps the fastest (synchronous)
     int autotune(int frames, const void* echo, const double* velocity, const long long* frame_row, float* img,
                  const long long* frame_img, const isar_options& opt, hipStream_t s, int repeats, int* chosen);
@@ -99,14 +111,10 @@ struct Context {
 };
 
 int next_pow2_ref(int n);   // common.cu:521-529
+// chirp-z transform lengths with a plan for M pulses: nextPow2(2M - 1) first (the reference's
+// length, mtrc.cu:125-172), then any shorter 3 * 2^k >= 2M - 1; returns how many (at most 2)
+int czt_length_candidates(int M, int* out);
 size_t workspace_bytes(const isar_radar_params& p, int frames);
-// tables shared by every frame of a context: chirp-z filter spectra and chirps, range window
-struct SharedTables {
-    float2* czt_table;
-    float2* czt_pre;
-    float2* czt_post;
-    float* win_range;
-};
 size_t carve_workspace(const isar_radar_params& p, int frames, char* base, Job* J, SharedTables* tables);
 int validate_params(const isar_radar_params& p);
 

@@ -215,6 +215,17 @@ ISAR_API int isar_sa_plan_groups(const isar_sa_plan* plan, int32_t* pulses, int3
     return static_cast<int>(plan->groups.size());
 }
 
+ISAR_API int isar_sa_plan_group_tuning(const isar_sa_plan* plan, int group, int32_t* tuning) {
+    if (!plan || !tuning || group < 0 || group >= static_cast<int>(plan->groups.size()))
+        return fail(ISAR_ERR_INVALID, "isar_sa_plan_group_tuning: bad arguments");
+    const isar::Context& c = plan->groups[group]->ctx;
+    tuning[0] = c.ra_chunk;
+    tuning[1] = c.af_mode;
+    tuning[2] = c.F;
+    tuning[3] = c.czt_threads;
+    return ISAR_OK;
+}
+
 ISAR_API int isar_sa_plan_run(isar_sa_plan* plan, const void* d_dwell, const double* d_velocity, float* d_img,
                               const isar_options* opt, int use_graph, void* stream) {
     if (!plan || !d_dwell || !d_img || !opt) return fail(ISAR_ERR_INVALID, "isar_sa_plan_run: null argument");

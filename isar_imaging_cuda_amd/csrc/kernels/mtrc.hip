@@ -22,12 +22,15 @@ namespace {
 constexpr int kRowE = 16;
 constexpr int kRowsPerBlock = 8;   // W = 512 rows: 32 threads per row transform
 
-template <int F>
+// Chirp-z column tiles: an F-point transform per range bin (F = 2^k with 16 elements per thread,
+// or 3 * 2^k with 12), NT_ threads per workgroup, CT = NT_ / TPF bins per workgroup.
+template <int F, int NT_ = 256>
 struct CztCfg {
-    static constexpr int E = 16;
+    static constexpr int E = (F % 3 == 0) ? 12 : 16;
     static constexpr int TPF = F / E;
-    static constexpr int NT = 256;
+    static constexpr int NT = NT_;
     static constexpr int CT = NT / TPF;
+    static_assert(F % E == 0 && NT % TPF == 0, "chirp-z tile");
     static constexpr int R0 = FftPlan<F, E>::R[0];
     static constexpr int RL = FftPlan<F, E>::R[FftPlan<F, E>::NP - 1];
     __device__ static int in(int lane, int e) { return fft_in_pos<F, E, R0>(lane, e); }
@@ -45,7 +48,7 @@ __device__ __forceinline__ int row_out(int lane, int e) {
 // Bluestein filter spectra: table[n][:] = FFT_F(v), v[t] = exp(+j pi alpha_n t^2 / M) for
 // |t| < M placed circularly, zero elsewhere. Phases reduced in FP64. One range bin per block.
 template <int F>
-__global__ __launch_bounds__(F / 16) void k_czt_table(int M, int W, double c0, double posa, float2* table) {
+__global__ __launch_bounds__(CztCfg<F>::TPF) void k_czt_table(int M, int W, double c0, double posa, float2* table) {
     HIP_DYNAMIC_SHARED(float2, lds);
     using C = CztCfg<F>;
     const int n = blockIdx.x, lane = threadIdx.x;
@@ -110,9 +113,9 @@ __global__ __launch_bounds__(kRowsPerBlock * W / kRowE) void k_mtrc_rows(Job J) 
 // Chirp-z transform along slow time for a tile of range bins (mtrc.cu:42-52, 125-192):
 // X[k] = exp(-j pi a k^2/M) / F * IFFT_F( FFT_F(y) * table_n )[k],
 // y[m] = x[m] exp(j pi a m (M - m) / M) for m < M. Also writes sum_k X[k] per bin.
-template <int F>
-__global__ __launch_bounds__(256) void k_mtrc_czt(Job J) {
-    using C = CztCfg<F>;
+template <int F, int NTW>
+__global__ __launch_bounds__(NTW) void k_mtrc_czt(Job J) {
+    using C = CztCfg<F, NTW>;
     constexpr int CT = C::CT, TPF = C::TPF, E = C::E, NT = C::NT;
     HIP_DYNAMIC_SHARED(float2, sm);
     const int tile = xcd_tile(blockIdx.x, gridDim.x), f = blockIdx.y, tid = threadIdx.x;
@@ -158,8 +161,7 @@ __global__ __launch_bounds__(256) void k_mtrc_czt(Job J) {
         if (lane == 0) J.colsum[static_cast<size_t>(f) * W + n] = csum;
     } else {
         // one transform spans TPF / 64 waves: wave sums first, then the transform's waves in LDS.
-        // The partials sit past the transpose area (M (CT + 1) entries), which the dynamic LDS
-        // size leaves free whenever a transform spans several waves (CT <= 2).
+        // The partials sit past the transpose area (M (CT + 1) entries); czt_lds counts them.
         float2* wave_part = sm + static_cast<size_t>(M) * (CT + 1);
         csum = group_sum2<64>(csum);
         if ((tid & 63) == 0) wave_part[tid / 64] = csum;
@@ -328,10 +330,11 @@ __global__ __launch_bounds__(256) void k_post_nomtrc(Job J) {
 
 // ------------------------------------------------------------------------------------------
 namespace {
-template <int F>
+template <int F, int NTW>
 size_t czt_lds(int M) {
-    using C = CztCfg<F>;
-    return std::max(static_cast<size_t>(M) * (C::CT + 1), static_cast<size_t>(C::CT) * lds_padded_size(F)) * sizeof(float2);
+    using C = CztCfg<F, NTW>;
+    return std::max(static_cast<size_t>(M) * (C::CT + 1) + NTW / 64, static_cast<size_t>(C::CT) * lds_padded_size(F)) *
+           sizeof(float2);
 }
 template <int M>
 size_t col_lds() {
@@ -344,7 +347,9 @@ size_t rows_lds() { return kRowsPerBlock * lds_padded_size(kRangeCut) * sizeof(f
     switch (F) {                                       \
     case 128: { constexpr int FF = 128; BODY; } break;   \
     case 256: { constexpr int FF = 256; BODY; } break;   \
+    case 384: { constexpr int FF = 384; BODY; } break;   \
     case 512: { constexpr int FF = 512; BODY; } break;   \
+    case 768: { constexpr int FF = 768; BODY; } break;   \
     case 1024: { constexpr int FF = 1024; BODY; } break; \
     case 2048: { constexpr int FF = 2048; BODY; } break; \
     default: break;                                    \
@@ -374,8 +379,9 @@ hipError_t configure_mtrc_kernels() {
     };
 #define ISAR_SET_F(FF)                                                                          \
     set(reinterpret_cast<const void*>(&k_czt_table<FF>), lds_padded_size(FF) * sizeof(float2)); \
-    set(reinterpret_cast<const void*>(&k_mtrc_czt<FF>), czt_lds<FF>(FF / 2));
-    ISAR_SET_F(128) ISAR_SET_F(256) ISAR_SET_F(512) ISAR_SET_F(1024) ISAR_SET_F(2048)
+    set(reinterpret_cast<const void*>(&k_mtrc_czt<FF, 256>), czt_lds<FF, 256>(FF / 2));         \
+    set(reinterpret_cast<const void*>(&k_mtrc_czt<FF, 512>), czt_lds<FF, 512>(FF / 2));
+    ISAR_SET_F(128) ISAR_SET_F(256) ISAR_SET_F(384) ISAR_SET_F(512) ISAR_SET_F(768) ISAR_SET_F(1024) ISAR_SET_F(2048)
 #undef ISAR_SET_F
 #define ISAR_SET_M(MM)                                                         \
     set(reinterpret_cast<const void*>(&k_col_window<MM>), col_lds<MM>());     \
@@ -396,16 +402,21 @@ void launch_mtrc_delta(const Job& J, hipStream_t s) {
                 s, J);
 }
 void launch_czt_table(int M, int W, int F, double c0, double posa, float2* table, hipStream_t s) {
-    ISAR_F_SWITCH(F, ISAR_LAUNCH(k_czt_table<FF>, dim3(W), dim3(FF / 16), lds_padded_size(FF) * sizeof(float2), s,
-                                 M, W, c0, posa, table));
+    ISAR_F_SWITCH(F, ISAR_LAUNCH(k_czt_table<FF>, dim3(W), dim3(CztCfg<FF>::TPF), lds_padded_size(FF) * sizeof(float2),
+                                 s, M, W, c0, posa, table));
 }
 void launch_mtrc_rows(const Job& J, hipStream_t s) {
     ISAR_LAUNCH(k_mtrc_rows<kRangeCut>, dim3((J.M + kRowsPerBlock - 1) / kRowsPerBlock, J.K),
                 dim3(kRowsPerBlock * kRangeCut / kRowE), rows_lds(), s, J);
 }
 void launch_mtrc_czt(const Job& J, hipStream_t s) {
-    ISAR_F_SWITCH(J.F, ISAR_LAUNCH(k_mtrc_czt<FF>, dim3(J.W / CztCfg<FF>::CT, J.K), dim3(CztCfg<FF>::NT),
-                                   czt_lds<FF>(J.M), s, J));
+    if (J.czt_threads == 512) {
+        ISAR_F_SWITCH(J.F, ISAR_LAUNCH((k_mtrc_czt<FF, 512>), dim3(J.W / CztCfg<FF, 512>::CT, J.K), dim3(512),
+                                       (czt_lds<FF, 512>(J.M)), s, J));
+    } else {
+        ISAR_F_SWITCH(J.F, ISAR_LAUNCH((k_mtrc_czt<FF, 256>), dim3(J.W / CztCfg<FF, 256>::CT, J.K), dim3(256),
+                                       (czt_lds<FF, 256>(J.M)), s, J));
+    }
 }
 void launch_col_window(const Job& J, hipStream_t s) {
     ISAR_PULSE_SWITCH2(J.M, ISAR_LAUNCH(k_col_window<MM>, dim3(J.W / ColCfg<MM>::CT, J.K), dim3(ColCfg<MM>::NT),

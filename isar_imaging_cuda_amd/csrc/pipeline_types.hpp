@@ -48,6 +48,7 @@ struct Job {
     int phi0_given;
     int ra_chunk;                 // rows per workgroup in the alignment update (2, 4, 8 or 16)
     int ra_split;                 // alignment update grids per level: 0 by batch size, 1 one, 2 two
+    int czt_threads;              // chirp-z workgroup width (256 or 512 threads)
     double hpc_k;                 // HPC phase in half-turns = hpc_k * v * n^2
     double czt_c0, czt_posa;      // alpha_n = 1 - c0 + posa * n
     // inputs

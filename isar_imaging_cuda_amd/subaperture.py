@@ -24,6 +24,7 @@ def _bind() -> C.CDLL:
     L.isar_sa_plan_run.argtypes = [vp, vp, vp, vp, vp, i, vp]
     L.isar_sa_plan_autotune.argtypes = [vp, vp, vp, vp, vp, i, vp]
     L.isar_sa_plan_groups.argtypes = [vp, vp, vp]
+    L.isar_sa_plan_group_tuning.argtypes = [vp, i, vp]
     L.isar_run_frames.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, vp]
     L.isar_set_ra_chunk.argtypes = [vp, i]
     L.isar_get_ra_chunk.argtypes = [vp]
@@ -89,6 +90,18 @@ class SubaperturePlan:
                                              C.c_void_p(img_ptr), C.byref(o), repeats,
                                              chosen.ctypes.data_as(C.c_void_p)), "isar_sa_plan_autotune")
         return chosen
+
+    def tuning(self) -> dict[int, dict[str, int]]:
+        """Per pulse count: the group's range-alignment chunk rows, autofocus form, chirp-z length
+        and chirp-z workgroup threads."""
+        out = {}
+        for g, M in enumerate(self.group_pulses):
+            t = np.zeros(4, np.int32)
+            _check(_bind().isar_sa_plan_group_tuning(self._h, g, t.ctypes.data_as(C.c_void_p)),
+                   "isar_sa_plan_group_tuning")
+            out[int(M)] = dict(ra_chunk=int(t[0]), autofocus_mode=int(t[1]), czt_length=int(t[2]),
+                               czt_threads=int(t[3]))
+        return out
 
     def image(self, flat: np.ndarray, i: int) -> np.ndarray:
         """Window i's image out of the flat output."""

@@ -53,12 +53,13 @@ class EmuFrame:
 
 def run(echo: np.ndarray, velocity: np.ndarray, radar: dict, opts: dict | None = None, phi0=None,
         timeout: float = 900.0, autofocus_mode: int | None = None, ra_split: int | None = None,
-        static_check: bool = False) -> list[EmuFrame]:
+        static_check: bool = False, czt_length: int | None = None, czt_threads: int | None = None) -> list[EmuFrame]:
     """echo: complex64 (K, M, N); velocity: (K, M). radar: band_width, fc, Tp, Fs.
     autofocus_mode: isar_autofocus_mode (0 auto, 1 one workgroup per frame, 2 split); ra_split:
     isar_set_ra_split (0 by batch size, 1 one grid, 2 two grids). static_check: the driver also
     images the batch and then its first frame through isar_run_device_static and fails unless both
-    equal the context's images bit for bit."""
+    equal the context's images bit for bit. czt_length, czt_threads: isar_set_czt_length and
+    isar_set_czt_threads."""
     if not DRIVER.exists():
         raise RuntimeError(f"{DRIVER} missing: build with isar_imaging_cuda_amd/build.py --emu")
     opts = opts or default_options()
@@ -80,9 +81,11 @@ def run(echo: np.ndarray, velocity: np.ndarray, radar: dict, opts: dict | None =
             f.write(vel.tobytes())
             if opts.get("warm_start"):
                 f.write(np.ascontiguousarray(phi0, dtype=np.complex64).tobytes())
-        extra = [] if autofocus_mode is None and ra_split is None and not static_check else [str(autofocus_mode or 0)]
-        extra += [] if ra_split is None and not static_check else [str(ra_split or 0)]
-        extra += ["1"] if static_check else []
+        # positional driver arguments; every one before the last given takes its default
+        args = [autofocus_mode, ra_split, 1 if static_check else None, czt_length, czt_threads]
+        defaults = [0, 0, 0, 0, 256]
+        last = max((i for i, a in enumerate(args) if a is not None), default=-1)
+        extra = [str(defaults[i] if args[i] is None else args[i]) for i in range(last + 1)]
         cmd = [str(DRIVER), str(fin), str(fout)] + extra
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:

@@ -5,7 +5,8 @@ level, and the rows of pair 0 apart from the other pairs, isar_set_ra_split). A 
 carries an odd group up a level (range_alignment.cu:34-73 generalised to any even M). Emulated
 device memory starts as all-ones bytes (NaN floats), so a kernel reading workspace it did not
 write in the same run would show; the static-workspace entry point (isar_run_device_static) must
-equal the context run bit for bit, on a first call and on a second, smaller one."""
+equal the context run bit for bit, on a first call and on a second, smaller one. The 192-pulse
+frame runs the chirp-z transform at the mixed-radix length 384 with 512-thread column tiles."""
 from __future__ import annotations
 
 import concurrent.futures as cf
@@ -22,7 +23,8 @@ pytestmark = pytest.mark.slow
 
 # a few autofocus iterations keep the emulated run to about a minute; the oracle runs as many
 ITERATIONS = 4
-CASES = [(128, 1, "c1, one grid"), (128, 2, "c1, two grids"), (192, 2, "M=192, two grids")]
+CASES = [(128, 1, None, "c1, one grid"), (128, 2, None, "c1, two grids"),
+         (192, 2, (384, 512), "M=192, two grids, chirp-z length 384, 512-thread tiles")]
 STATIC_CASE = (64, 2, "M=64 x 2 frames, static workspace")
 
 
@@ -31,12 +33,13 @@ def emu_driver():
     return B.build_emu(force=False)
 
 
-def _one(M, split):
+def _one(M, split, czt):
     cfg = synth.preset(1, echo_num=M, range_num=1024)
     echo, vel = synth.frame(cfg)
     radar = synth.radar_header(cfg)
+    length, threads = czt or (None, None)
     (fr,) = emu_runner.run(echo, vel, radar, emu_runner.default_options(iterations=ITERATIONS), ra_split=split,
-                           timeout=600)
+                           czt_length=length, czt_threads=threads, timeout=600)
     img_o, tr = oracle_run(echo, vel, radar, M, 1024, iterations=ITERATIONS)
     return fr, compare(fr.img, fr, img_o, tr, M), tr
 
@@ -59,7 +62,7 @@ def _static(M, frames):
 
 def test_emulated_kernels_match_oracle(emu_driver):
     with cf.ThreadPoolExecutor(max_workers=len(CASES) + 1) as ex:
-        futs = {name: ex.submit(_one, M, split) for M, split, name in CASES}
+        futs = {name: ex.submit(_one, M, split, czt) for M, split, czt, name in CASES}
         fstatic = ex.submit(_static, STATIC_CASE[0], STATIC_CASE[1])
         res = {name: f.result() for name, f in futs.items()}
         static_reps = fstatic.result()

@@ -205,3 +205,69 @@ def test_ra_update_launch_forms_agree(cid, over):
     (a, ta), (b, tb) = out
     assert np.array_equal(ta.ra_lag, tb.ra_lag) and np.array_equal(ta.phi, tb.phi)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("M,N", [(64, 1024), (128, 1024), (192, 1024), (256, 2048), (384, 1024), (512, 1024),
+                                 (1024, 2048)])
+def test_czt_lengths_and_tiles(M, N):
+    """Every chirp-z length the context accepts (nextPow2(2M - 1), and 384 or 768 for 192 and 384
+    pulses) against the oracle's chirp-z transform (mtrc.cu:125-192, any length gives the same
+    transform), and both column-tile widths bit for bit; the lengths agree to FP32 rounding."""
+    cfg, echo, vel, radar = _frame(1, echo_num=M, range_num=N)
+    params = isar.RadarParams(M, N, radar["band_width"], radar["fc"], radar["Tp"], radar["Fs"])
+    img_o, tr = oracle_run(echo, vel, radar, M, N)
+    with isar.Imager(params, max_frames=1) as im:
+        lengths = im.czt_lengths()
+        p2 = 1 << (2 * M - 1).bit_length()
+        assert lengths[0] == p2 and im.czt_length == p2
+        assert lengths[1:] == ([384] if M == 192 else [768] if M == 384 else [])
+        with pytest.raises(isar.ImagingError):
+            im.set_czt_length(p2 * 2)
+        out = {}
+        for length in lengths:
+            im.set_czt_length(length)
+            for threads in (256, 512):
+                print(f"[case] chirp-z M={M} N={N} length {length} threads {threads}", flush=True)
+                im.set_czt_threads(threads)
+                img = im.run_host(echo, vel, isar.Options())
+                t = im.trace(0)
+                rep = compare(img, t, img_o, tr, M)
+                assert rep.ok(), f"length {length} threads {threads}: {rep}"
+                out[length, threads] = img
+        for length in lengths:
+            assert np.array_equal(out[length, 256], out[length, 512]), f"length {length}: tile widths differ"
+        ref = out[lengths[0], 256]
+        for length in lengths[1:]:
+            rel = np.linalg.norm(out[length, 256] - ref) / np.linalg.norm(ref)
+            print(f"length {length} vs {lengths[0]}: rel L2 {rel:.1e}")
+            assert rel < 1e-5
+
+
+@pytest.mark.parametrize("M", [192, 256])
+def test_autotune_keeps_a_valid_configuration(M):
+    """isar_autotune over chunk rows, autofocus form, chirp-z length and column tile: the kept
+    configuration is one of the candidates and images within the parity bars."""
+    import torch
+    K = 4
+    cfgs = [synth.preset(1, echo_num=M, seed=900 + k) for k in range(K)]
+    frames = [synth.frame(c) for c in cfgs]
+    radar = synth.radar_header(cfgs[0])
+    N = cfgs[0].range_num
+    echo, vel = np.stack([f[0] for f in frames]), np.stack([f[1] for f in frames])
+    params = isar.RadarParams(M, N, radar["band_width"], radar["fc"], radar["Tp"], radar["Fs"])
+    dev = torch.device("cuda:0")
+    d_echo = torch.from_numpy(echo.view(np.float32)).to(dev)
+    d_vel = torch.from_numpy(vel).to(dev)
+    d_img = torch.zeros((K, M, 512), dtype=torch.float32, device=dev)
+    print(f"[case] autotune M={M} x {K} frames", flush=True)
+    with isar.Imager(params, max_frames=K) as im:
+        s = torch.cuda.current_stream().cuda_stream
+        rows = im.autotune(d_echo.data_ptr(), d_vel.data_ptr(), d_img.data_ptr(), K, repeats=2, stream=s)
+        print(f"chunk {rows}, autofocus {im.autofocus_mode}, chirp-z {im.czt_length} x {im.czt_threads}")
+        assert rows in (2, 4, 8, 16)
+        assert im.czt_length in im.czt_lengths() and im.czt_threads in (256, 512)
+        img = im.run_host(echo, vel, isar.Options())
+        for k in range(K):
+            img_o, tr = oracle_run(echo[k], vel[k], radar, M, N)
+            rep = compare(img[k], im.trace(k), img_o, tr, M)
+            assert rep.ok(), f"frame {k}: {rep}"

@@ -73,6 +73,11 @@ def test_windows_match_oracle_and_autotune():
     print(f"[case] autotuned chunk rows per pulse count: {dict(zip(plan.group_pulses.tolist(), chosen.tolist()))}",
           flush=True)
     assert set(chosen.tolist()) <= {2, 4, 8, 16}
+    tuning = plan.tuning()
+    print("plan tuning per pulse count:", tuning, flush=True)
+    for M, t in tuning.items():
+        assert t["czt_length"] in ({384, 512} if M == 192 else {768, 1024} if M == 384 else {1 << (2 * M - 1).bit_length()})
+        assert t["czt_threads"] in (256, 512) and t["autofocus_mode"] in (1, 2, 3)
     plan.run(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), graph=True, stream=s)
     torch.cuda.synchronize()
     out = img.cpu().numpy()
