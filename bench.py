@@ -39,15 +39,15 @@ def _ra_levels(M: int) -> list[int]:
     return out
 
 
-def stage_bytes_per_frame(stage: str, M: int, N: int, W: int, U: float) -> float:
+def stage_bytes_per_frame(stage: str, M: int, N: int, W: int, U: float, chunk: int = 16) -> float:
     """Algorithmic HBM bytes of one frame for one launch of a kernel family, averaged over the
     launches of that family in one run (range-alignment families launch once per merge level).
 
     ra_pair0   read every echo row (8N), store its spectrum S0 (8N), one merged energy row per
                pair (4N), the two final profiles of rows 0-1 (8N each) and their column sums (4N)
     ra_upd(s)  read S0 of the M/2 rows that move (8N each), one partial energy row (4N) per
-               chunk of R = min(s, 8) moved rows, final profiles of pair 0's s moved rows (8N each)
-               and their per-chunk column sums (4N)
+               chunk of R = min(s, chunk) moved rows, final profiles of pair 0's s moved rows (8N
+               each) and their per-chunk column sums (4N)
     ra_corr(s) per pair: two group energies read (4N each) plus the chunk partials of each
                group's second half, the first group's energy written back (4N)
     """
@@ -56,14 +56,14 @@ def stage_bytes_per_frame(stage: str, M: int, N: int, W: int, U: float) -> float
     if stage == "ra_upd":
         per = []
         for s in _ra_levels(M):
-            R = min(s, 8)
+            R = min(s, chunk)
             per.append((M / 2) * 8.0 * N + (M / 2 / R) * 4.0 * N + s * 8.0 * N + (s / R) * 4.0 * N)
         return sum(per) / len(per)
     if stage == "ra_corr":
         per = []
         for s in _ra_levels(M):
             pairs = M // (2 * s)
-            parts = 0 if s == 2 else (s // 2) // min(s // 2, 8)
+            parts = 0 if s == 2 else (s // 2) // min(s // 2, chunk)
             per.append(pairs * (2 * 4.0 * N + 2 * parts * 4.0 * N + 4.0 * N))
         return sum(per) / len(per)
     return {
@@ -382,14 +382,17 @@ def host_transfer(echo, img, ms_per_step: float, stream, max_frames: int = 256) 
             "pcie_inclusive_frames_per_s": round(n / ((h2d + ms_per_step * n / K + d2h) / 1e3), 1)}
 
 
-def load_traffic(kernel: str, frames: int) -> dict | None:
+def load_traffic(kernel: str, frames: int, chunk: int) -> dict | None:
     """HBM traffic per launch of `kernel` at this workload from a committed PMC summary
     (profiles/traffic_<kernel>_256x4096.json, written from two rocprofv3 --pmc runs and scaled by
-    the same counters read on a plain copy). None when no summary exists."""
+    the same counters read on a plain copy). None when no summary exists or when it was measured
+    at another range-alignment chunk size."""
     path = ROOT / "profiles" / f"traffic_{kernel}_256x4096.json"
     if not path.exists():
         return None
     d = json.loads(path.read_text())
+    if d.get("ra_chunk_rows") != chunk:
+        return None
     per_frame = d["hbm_bytes_per_launch"] / d["frames_per_launch"]
     return {"bytes": per_frame * frames, "source": path.name}
 
@@ -407,7 +410,7 @@ def main() -> int:
     ap.add_argument("--cpu-frames", type=int, default=64)
     ap.add_argument("--ra-chunk", type=int, default=0, help="fix the range-alignment chunk size (no autotuning)")
     ap.add_argument("--autotune", action="store_true",
-                    help="let the autotuner pick the range-alignment chunk size on this batch (default: 8 rows, "
+                    help="let the autotuner pick the range-alignment chunk size on this batch (default: 16 rows, "
                          "the size the committed traffic measurement used)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-vs-plain iteration count")
     ap.add_argument("--no-subapertures", action="store_true", help="skip the sub-aperture plan measurement")
@@ -467,7 +470,8 @@ def main() -> int:
         ra_chunk = im.autotune(echo.data_ptr(), vel.data_ptr(), img.data_ptr(), K, opts, repeats=2,
                                stream=stream.cuda_stream)
     else:
-        ra_chunk = 8
+        ra_chunk = 16
+        im.set_ra_chunk(ra_chunk)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -501,9 +505,9 @@ def main() -> int:
     im.probe_enable(-1)
     avg_launch_ms = probe_ms / max(probe_n, 1)
     stage_us = gpu_stage_times(im, step, K) if not args.no_stages else None
-    alg = stage_bytes_per_frame(probe, M, N, W, U) * K
+    alg = stage_bytes_per_frame(probe, M, N, W, U, ra_chunk) * K
     achieved_gbs = alg / (avg_launch_ms / 1e3) / 1e9 if probe_n else 0.0
-    traffic = load_traffic(probe, K)
+    traffic = load_traffic(probe, K, ra_chunk)
     frame_bytes = algorithmic_bytes_per_frame(M, N, W, U)
     out = {
         "metric": "frames/s @256x4096 cf32, % HBM roofline; learned vs plain at equal entropy",
