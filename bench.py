@@ -409,6 +409,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-frames", type=int, default=64)
     ap.add_argument("--ra-chunk", type=int, default=0, help="fix the range-alignment chunk size (no autotuning)")
+    ap.add_argument("--ra-split", type=int, default=0, help="range-alignment update launch form (isar_set_ra_split)")
     ap.add_argument("--autotune", action="store_true",
                     help="let the autotuner pick the range-alignment chunk size on this batch (default: 16 rows, "
                          "the size the committed traffic measurement used)")
@@ -472,6 +473,8 @@ def main() -> int:
     else:
         ra_chunk = 16
         im.set_ra_chunk(ra_chunk)
+    if args.ra_split:
+        im.set_ra_split(args.ra_split)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
