@@ -580,6 +580,53 @@ struct FftPassesCached {
     }
 };
 
+// All twiddle powers w^r (r < E) of the passes that hold one radix group per thread, formed once
+// per thread from the cached bases (for the + direction; the - direction uses their conjugates,
+// which are bit for bit the powers of the conjugated bases). For kernels that run many
+// transforms and have the registers to keep E complex values per such pass live across them.
+template <int N, int E>
+struct FftTwiddlePowers {
+    float2 w[5][E];   // rows of the other passes are never formed or read
+};
+
+template <int N, int E, int DIR, int P, int Ns>
+struct FftPassesPow {
+    using Plan = FftPlan<N, E>;
+    __device__ __forceinline__ static void run(float2 (&x)[E], float2* lds, int tid, const FftTwiddlePowers<N, E>& pw) {
+        constexpr int R = Plan::R[P];
+        if constexpr (E == R) {
+            if constexpr (Ns > 1) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 w = pw.w[P][r];
+                    x[r] = cmul(x[r], DIR < 0 ? make_float2(w.x, -w.y) : w);
+                }
+            }
+            Dft<R, DIR>::run(x);
+        } else {
+            stockham_compute<N, E, R, DIR, Ns>(x, tid);
+        }
+        if constexpr (P + 1 < Plan::NP) {
+            constexpr int RN = Plan::R[P + 1];
+            stockham_store<N, E, R, Ns>(x, lds, tid);
+            __syncthreads();
+            stockham_load<N, E, RN>(x, lds, tid);
+            __syncthreads();
+            FftPassesPow<N, E, DIR, P + 1, Ns * R>::run(x, lds, tid, pw);
+        }
+    }
+};
+
+template <int N, int E, int P, int Ns>
+__device__ __forceinline__ void fft_twiddle_powers_fill(FftTwiddlePowers<N, E>& pw, const FftTwiddles<N, E>& tw) {
+    using Plan = FftPlan<N, E>;
+    if constexpr (P < Plan::NP) {
+        constexpr int R = Plan::R[P];
+        if constexpr (Ns > 1 && R == E) twiddle_powers_from<R>(pw.w[P], tw.w1[P], tw.w4[P]);
+        fft_twiddle_powers_fill<N, E, P + 1, Ns * R>(pw, tw);
+    }
+}
+
 template <int N, int E, int P, int Ns>
 __device__ __forceinline__ void fft_twiddles_fill(FftTwiddles<N, E>& tw, int tid) {
     using Plan = FftPlan<N, E>;
@@ -617,6 +664,17 @@ __device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid) {
 template <int N, int E, int DIR>
 __device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid, const FftTwiddles<N, E>& tw) {
     FftPassesCached<N, E, DIR, 0, 1>::run(x, lds, tid, tw);
+}
+// The same transform with all twiddle powers precomputed (fft_twiddle_powers).
+template <int N, int E>
+__device__ __forceinline__ FftTwiddlePowers<N, E> fft_twiddle_powers(const FftTwiddles<N, E>& tw) {
+    FftTwiddlePowers<N, E> pw;
+    fft_twiddle_powers_fill<N, E, 0, 1>(pw, tw);
+    return pw;
+}
+template <int N, int E, int DIR>
+__device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid, const FftTwiddlePowers<N, E>& pw) {
+    FftPassesPow<N, E, DIR, 0, 1>::run(x, lds, tid, pw);
 }
 
 template <int N, int E>

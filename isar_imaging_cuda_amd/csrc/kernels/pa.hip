@@ -240,7 +240,8 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     float2* buf = bufs + g * lds_padded_size(M);
     const float invM = 1.0f / static_cast<float>(M);
     const bool resident = U <= CF;   // one chunk: the columns stay in registers
-    const FftTwiddles<M, E> tw = fft_twiddles<M, E>(lane);
+    // the twiddle powers stay in registers for all iterations' transforms
+    const FftTwiddlePowers<M, E> tw = fft_twiddle_powers<M, E>(fft_twiddles<M, E>(lane));
     float2 s[E];
     for (int it = 0; it < J.iterations; ++it) {
         float2 acc[RPT];

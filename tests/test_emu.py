@@ -6,7 +6,9 @@ carries an odd group up a level (range_alignment.cu:34-73 generalised to any eve
 device memory starts as all-ones bytes (NaN floats), so a kernel reading workspace it did not
 write in the same run would show; the static-workspace entry point (isar_run_device_static) must
 equal the context run bit for bit, on a first call and on a second, smaller one. The 192-pulse
-frame runs the chirp-z transform at the mixed-radix length 384 with 512-thread column tiles."""
+frame runs the chirp-z transform at the mixed-radix length 384 with 512-thread column tiles. The
+first c1 case runs the one-workgroup autofocus and the second the split form, and the two images
+must agree bit for bit."""
 from __future__ import annotations
 
 import concurrent.futures as cf
@@ -23,8 +25,9 @@ pytestmark = pytest.mark.slow
 
 # a few autofocus iterations keep the emulated run to about a minute; the oracle runs as many
 ITERATIONS = 4
-CASES = [(128, 1, None, "c1, one grid"), (128, 2, None, "c1, two grids"),
-         (192, 2, (384, 512), "M=192, two grids, chirp-z length 384, 512-thread tiles")]
+# (pulses, ra_split, (chirp-z length, threads) or None, autofocus mode or None, name)
+CASES = [(128, 1, None, 1, "c1, one grid"), (128, 2, None, 2, "c1, two grids"),
+         (192, 2, (384, 512), None, "M=192, two grids, chirp-z length 384, 512-thread tiles")]
 STATIC_CASE = (64, 2, "M=64 x 2 frames, static workspace")
 
 
@@ -33,13 +36,13 @@ def emu_driver():
     return B.build_emu(force=False)
 
 
-def _one(M, split, czt):
+def _one(M, split, czt, af):
     cfg = synth.preset(1, echo_num=M, range_num=1024)
     echo, vel = synth.frame(cfg)
     radar = synth.radar_header(cfg)
     length, threads = czt or (None, None)
-    (fr,) = emu_runner.run(echo, vel, radar, emu_runner.default_options(iterations=ITERATIONS), ra_split=split,
-                           czt_length=length, czt_threads=threads, timeout=600)
+    (fr,) = emu_runner.run(echo, vel, radar, emu_runner.default_options(iterations=ITERATIONS), autofocus_mode=af,
+                           ra_split=split, czt_length=length, czt_threads=threads, timeout=600)
     img_o, tr = oracle_run(echo, vel, radar, M, 1024, iterations=ITERATIONS)
     return fr, compare(fr.img, fr, img_o, tr, M), tr
 
@@ -62,7 +65,7 @@ def _static(M, frames):
 
 def test_emulated_kernels_match_oracle(emu_driver):
     with cf.ThreadPoolExecutor(max_workers=len(CASES) + 1) as ex:
-        futs = {name: ex.submit(_one, M, split, czt) for M, split, czt, name in CASES}
+        futs = {name: ex.submit(_one, M, split, czt, af) for M, split, czt, af, name in CASES}
         fstatic = ex.submit(_static, STATIC_CASE[0], STATIC_CASE[1])
         res = {name: f.result() for name, f in futs.items()}
         static_reps = fstatic.result()
