@@ -558,13 +558,26 @@ struct FftPasses {
 };
 
 // Same passes with the twiddle bases taken from a per-thread cache (one group per thread).
-template <int N, int E, int DIR, int P, int Ns>
+// The radix-16 pass at stride 16 needs only the 256 twiddles exp(j 2 pi m / 256): a kernel that
+// runs many transforms may keep them in LDS (fft_tw16_table_fill) and read them there
+// (TAB = true) instead of forming the 15 powers from the cached bases at every transform.
+template <int N, int E, int DIR, int P, int Ns, bool TAB = false>
 struct FftPassesCached {
     using Plan = FftPlan<N, E>;
-    __device__ __forceinline__ static void run(float2 (&x)[E], float2* lds, int tid, const FftTwiddles<N, E>& tw) {
+    __device__ __forceinline__ static void run(float2 (&x)[E], float2* lds, int tid, const FftTwiddles<N, E>& tw,
+                                               const float2* tab = nullptr) {
         constexpr int R = Plan::R[P];
         if constexpr (E == R) {   // one radix group per thread: twiddles from the cache
-            if constexpr (Ns > 1) stockham_twiddle_cached<N, E, DIR, Ns, R>(x, tw.w1[P], tw.w4[P]);
+            if constexpr (TAB && Ns == 16 && R == 16) {
+                const int k = opaque(tid) & 15;
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    const float2 w = tab[k * r];
+                    x[r] = cmul(x[r], DIR < 0 ? make_float2(w.x, -w.y) : w);
+                }
+            } else if constexpr (Ns > 1) {
+                stockham_twiddle_cached<N, E, DIR, Ns, R>(x, tw.w1[P], tw.w4[P]);
+            }
             Dft<R, DIR>::run(x);
         } else {
             stockham_compute<N, E, R, DIR, Ns>(x, tid);
@@ -575,10 +588,16 @@ struct FftPassesCached {
             __syncthreads();
             stockham_load<N, E, RN>(x, lds, tid);
             __syncthreads();
-            FftPassesCached<N, E, DIR, P + 1, Ns * R>::run(x, lds, tid, tw);
+            FftPassesCached<N, E, DIR, P + 1, Ns * R, TAB>::run(x, lds, tid, tw, tab);
         }
     }
 };
+
+// tab[m] = exp(j 2 pi m / 256), m < 256, each from an FP64 sincospi rounded once; every thread of
+// the workgroup calls it, and the transforms' first LDS barrier orders it before any use.
+__device__ __forceinline__ void fft_tw16_table_fill(float2* tab, int tid, int nthreads) {
+    for (int m = tid; m < 256; m += nthreads) tab[m] = expjpi_rounded(static_cast<double>(m) / 128.0);
+}
 
 // All twiddle powers w^r (r < E) of the passes that hold one radix group per thread, formed once
 // per thread from the cached bases (for the + direction; the - direction uses their conjugates,
@@ -664,6 +683,12 @@ __device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid) {
 template <int N, int E, int DIR>
 __device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid, const FftTwiddles<N, E>& tw) {
     FftPassesCached<N, E, DIR, 0, 1>::run(x, lds, tid, tw);
+}
+// ... and the stride-16 radix-16 pass's twiddles from an LDS table (fft_tw16_table_fill).
+template <int N, int E, int DIR>
+__device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid, const FftTwiddles<N, E>& tw,
+                                    const float2* tab) {
+    FftPassesCached<N, E, DIR, 0, 1, true>::run(x, lds, tid, tw, tab);
 }
 // The same transform with all twiddle powers precomputed (fft_twiddle_powers).
 template <int N, int E>

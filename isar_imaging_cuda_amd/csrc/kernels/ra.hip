@@ -24,6 +24,13 @@ __device__ __forceinline__ int out_pos(int tid, int e) {
     return fft_out_pos<N, kE, FftPlan<N, kE>::R[FftPlan<N, kE>::NP - 1]>(tid, e);
 }
 
+// Dynamic LDS of the row kernels: the transform buffer (lds_padded_size(N) float2), 64 floats of
+// reductions, then the 256 twiddles of the stride-16 radix-16 pass (fft_tw16_table_fill).
+template <int N>
+__device__ __forceinline__ float2* tw16_table(float2* lds) {
+    return reinterpret_cast<float2*>(reinterpret_cast<char*>(lds + lds_padded_size(N)) + 64 * sizeof(float));
+}
+
 // Spectrum of row m of frame f with HPC (FP64 phase), window and (-1)^n centring applied, in
 // the first-pass layout (range_alignment.cu:13-18, high_speed_compensation.cu:4-54). With
 // if_hrrp the STRETCH rows carry the window twice (getHRRP windows them in place first,
@@ -140,12 +147,12 @@ __device__ __forceinline__ void group_energy(float (&a)[kE], const Job& J, int f
 // correlation comes from one complex FFT of gA + j gB. Every thread gets the argmax.
 template <int N>
 __device__ int correlation_argmax(const float (&a)[kE], const float (&b)[kE], float2* lds, float* redv, int* redi,
-                                  int tid, const FftTwiddles<N, kE>& tw) {
+                                  int tid, const FftTwiddles<N, kE>& tw, const float2* tab) {
     constexpr int NT = N / kE;
     float2 x[kE];
 #pragma unroll
     for (int e = 0; e < kE; ++e) x[e] = make_float2(sqrt_hw(a[e]), sqrt_hw(b[e]));
-    fft<N, kE, -1>(x, lds, tid, tw);
+    fft<N, kE, -1>(x, lds, tid, tw, tab);
 #pragma unroll
     for (int e = 0; e < kE; ++e) lds[lds_pad(out_pos<N>(tid, e))] = x[e];
     __syncthreads();
@@ -158,7 +165,7 @@ __device__ int correlation_argmax(const float (&a)[kE], const float (&b)[kE], fl
         x[e] = cmulc(ga, gb);
     }
     __syncthreads();
-    fft<N, kE, +1>(x, lds, tid, tw);
+    fft<N, kE, +1>(x, lds, tid, tw, tab);
     float best = -1.0f;
     int bi = 0;
 #pragma unroll
@@ -315,6 +322,8 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
     float* dst = J.egrp + (static_cast<size_t>(f) * M + ra) * N;
     float* ap = J.arp_part + static_cast<size_t>(f) * (M / 2) * N;
     const FftTwiddles<N, kE> tw = fft_twiddles<N, kE>(tid);
+    float2* const tab = tw16_table<N>(lds);
+    fft_tw16_table_fill(tab, tid, NT);
     float2 x[kE];
     float ea[kE];
     float mx = 0.0f;
@@ -327,7 +336,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
 #pragma unroll
         for (int e = 0; e < kE; ++e) s0a[in_pos<N>(t, e)] = x[e];
     }
-    fft<N, kE, +1>(x, lds, tid, tw);
+    fft<N, kE, +1>(x, lds, tid, tw, tab);
     {
         const int t = opaque(tid);
 #pragma unroll
@@ -355,7 +364,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
 #pragma unroll
         for (int e = 0; e < kE; ++e) s0b[in_pos<N>(t, e)] = x[e];
     }
-    fft<N, kE, +1>(x, lds, tid, tw);
+    fft<N, kE, +1>(x, lds, tid, tw, tab);
     // Above 4096 cells the two energy rows do not stay in registers across the correlation's
     // transforms: row 2p's is in its merged-energy row already, and row 2p+1's goes to that
     // row's energy slot, which no group at stride >= 2 uses (group heads are even rows). Each
@@ -375,7 +384,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
 #pragma unroll
     for (int e = 0; e < kE; ++e) { ta[e] = x[e].x; tb[e] = x[e].y; }
     const size_t idx = static_cast<size_t>(f) * (M - 1) + p;
-    const int bi = correlation_argmax<N>(ta, tb, lds, redv, redi, tid, tw);
+    const int bi = correlation_argmax<N>(ta, tb, lds, redv, redi, tid, tw, tab);
     double lag;
     if constexpr (kStash) {
         const int t = opaque(tid);
@@ -390,7 +399,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
     }
     // shift row 2p+1 by the lag and merge (range_alignment.cu:66-69)
     load_row_shifted<N>(x, J, f, rb, lag, tid);
-    fft<N, kE, +1>(x, lds, tid, tw);
+    fft<N, kE, +1>(x, lds, tid, tw, tab);
     {
         const int t = opaque(tid);
 #pragma unroll
@@ -431,6 +440,8 @@ __global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base)
     float a[kE], b[kE];
     group_energy<N>(a, J, f, gA, s, tid);
     const FftTwiddles<N, kE> tw = fft_twiddles<N, kE>(tid);
+    float2* const tab = tw16_table<N>(lds);
+    fft_tw16_table_fill(tab, tid, N / kE);
     {   // keep the first group's energy for the next level (same head row, same thread)
         float* dst = J.egrp + (static_cast<size_t>(f) * M + static_cast<size_t>(gA) * s) * N;
 #pragma unroll
@@ -439,7 +450,7 @@ __global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base)
     if (unpaired) return;
     group_energy<N>(b, J, f, gA + 1, s, tid);
     const size_t idx = static_cast<size_t>(f) * (M - 1) + pair_base + p;
-    const int bi = correlation_argmax<N>(a, b, lds, redv, redi, tid, tw);
+    const int bi = correlation_argmax<N>(a, b, lds, redv, redi, tid, tw, tab);
     // read both energies again rather than hold them across the transforms (the first from the
     // row this thread just wrote)
     {
@@ -484,6 +495,8 @@ __global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) voi
     for (int e = 0; e < kE; ++e) acc[e] = 0.0f;
     float2 x[kE];
     const FftTwiddles<N, kE> tw = fft_twiddles<N, kE>(tid);
+    float2* const tab = tw16_table<N>(lds);
+    fft_tw16_table_fill(tab, tid, N / kE);
     // column sums of |profile| over the final rows of this chunk: kept in registers by the
     // final-row grid up to 4096 cells (above, its workgroup size leaves no room) and stored once,
     // otherwise accumulated in place (each location is read back only by the thread that wrote
@@ -496,7 +509,7 @@ __global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) voi
     for (int m = r0; m < r1; ++m) {
         const double L = J.lagsum[static_cast<size_t>(f) * M + m] + lag;
         load_row_shifted<N>(x, J, f, m, L, tid);
-        fft<N, kE, +1>(x, lds, tid, tw);
+        fft<N, kE, +1>(x, lds, tid, tw, tab);
 #pragma unroll
         for (int e = 0; e < kE; ++e) acc[e] += cabs2(x[e]) * inv;
         if (final_rows) {   // S0 of row m was read before the transform's first barrier
@@ -543,7 +556,7 @@ __global__ __launch_bounds__(256) void k_range_window(int N, float* win) {
 // ------------------------------------------------------------------------------------------
 namespace {
 template <int N>
-size_t ra_lds() { return lds_padded_size(N) * sizeof(float2) + 64 * sizeof(float); }
+size_t ra_lds() { return lds_padded_size(N) * sizeof(float2) + 64 * sizeof(float) + 256 * sizeof(float2); }
 
 #define ISAR_RANGE_SWITCH(N, BODY)                     \
     switch (N) {                                       \
