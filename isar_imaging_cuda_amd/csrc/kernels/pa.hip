@@ -188,6 +188,15 @@ struct AfCfg {
     static_assert(CF % Q == 0, "sub-chunks must tile a chunk");
 };
 
+// One term conj(S) IFFT(G |G|) / M of the per-pulse sums. Only the unit phasor of the sum is
+// kept, so for a power-of-two M the 1/M is left out: scaling by 2^-k commutes with every rounding
+// of the sum, its magnitude and the division, and the phases are the same bit for bit.
+template <int M>
+__device__ __forceinline__ float2 af_term(float2 s, float2 x, float invM) {
+    if constexpr ((M & (M - 1)) == 0) return cconjmul(s, x);
+    else return cscale(cconjmul(s, x), invM);
+}
+
 template <int M>
 __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     using A = AfCfg<M>;
@@ -267,7 +276,7 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
             for (int e = 0; e < E; ++e) {
                 const int pos = C::out(lane, e);
                 const float2 sv = C::same_layout ? s[e] : (valid ? col[pos] : make_float2(0.0f, 0.0f));
-                buf[pos] = cscale(cconjmul(sv, x[e]), invM);
+                buf[pos] = af_term<M>(sv, x[e], invM);
             }
             __syncthreads();
             const int ncol = min(CF, U - c0);
@@ -320,7 +329,8 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
 // to fill the GPU with one workgroup per frame. k_autofocus with no iterations gathers the
 // selected bins and forms the starting phase; then iteration `it` is a launch of k_af_cols:
 // workgroup (j, f) takes the frame's current phase, transforms sub-chunk j of the selected bins
-// and writes its per-pulse sums of conj(S) IFFT(G |G|) / M into the iteration's half of af_part.
+// and writes its per-pulse sums of conj(S) IFFT(G |G|) (/ M unless M is a power of two, af_term)
+// into the iteration's half of af_part.
 // The current phase is either formed by every workgroup alike from the previous iteration's sums
 // (FOLD: one launch per iteration) or read from J.phi, which k_af_phase writes after each
 // iteration (two launches per iteration, no repeated work). The transforms and the summation
@@ -389,7 +399,7 @@ __global__ __launch_bounds__(AfCfg<M>::SUB * AfCfg<M>::TPF) void k_af_cols(Job J
     for (int e = 0; e < E; ++e) {
         const int pos = C::out(lane, e);
         const float2 sv = C::same_layout ? s[e] : (valid ? col[pos] : make_float2(0.0f, 0.0f));
-        buf[pos] = cscale(cconjmul(sv, x[e]), invM);
+        buf[pos] = af_term<M>(sv, x[e], invM);
     }
     __syncthreads();
     const int ncol = min(SUB, U - c0);
