@@ -330,11 +330,13 @@ def large_batch(frames: int = 8, steps: int = 3, learned_frames: int = 16, two_s
 
 def single_frame(params, echo, vel, stream, runs: int = 20) -> dict:
     """Latency of one frame alone (a batch of one, device-resident), the drop-in's unit of work:
-    replayed as a HIP graph (graph mode, as the drop-in session runs) and launched directly."""
+    with the drop-in session's two-row alignment chunks, replayed as a HIP graph (graph mode, as
+    the drop-in session runs) and launched directly."""
     import torch
 
     import isar_imaging_cuda_amd as isar
     im = isar.Imager(params, max_frames=1)
+    im.set_ra_chunk(2)   # as the drop-in session images its one frame per call
     img1 = torch.empty((1, params.echo_num, W), dtype=torch.float32, device=echo.device)
     run = lambda: im.run_device(echo.data_ptr(), vel.data_ptr(), img1.data_ptr(), 1, isar.Options(),
                                 stream=stream.cuda_stream)
