@@ -194,11 +194,20 @@ def learned_vs_plain(frames: int = 1000) -> dict:
     return res
 
 
-def subapertures(count: int = 64, replays: int = 10, autotune: bool = True) -> dict:
+# per pulse count of the sub-aperture plan: chunk rows, autofocus form, chirp-z length and
+# threads, as the plan's autotuner chose them for this workload on an MI355X
+SA_TUNING = {128: dict(ra_chunk=2, autofocus_mode=3, czt_length=256, czt_threads=256),
+             192: dict(ra_chunk=4, autofocus_mode=3, czt_length=512, czt_threads=256),
+             256: dict(ra_chunk=4, autofocus_mode=1, czt_length=512, czt_threads=256),
+             384: dict(ra_chunk=4, autofocus_mode=3, czt_length=1024, czt_threads=256),
+             512: dict(ra_chunk=4, autofocus_mode=3, czt_length=1024, czt_threads=256)}
+
+
+def subapertures(count: int = 64, replays: int = 10, tuned: bool = True) -> dict:
     """64 sliding windows of {128, 192, 256, 384, 512} pulses of one 8192 x 4096 complex64 dwell
-    imaged by one sub-aperture plan (graph replay after autotuning; without it, the default
-    chunk size and autofocus form, so that a kernel trace holds only the capture run and the
-    replays)."""
+    imaged by one sub-aperture plan, replayed as a graph: with the tuning table SA_TUNING set
+    through isar_sa_plan_set_group_tuning, or (tuned=False) the default chunk size and autofocus
+    form. The plan's autotuner is not run here."""
     import torch
 
     import isar_imaging_cuda_amd as isar
@@ -215,8 +224,8 @@ def subapertures(count: int = 64, replays: int = 10, autotune: bool = True) -> d
     d_echo = torch.from_numpy(echo.view(np.float32)).to(dev)
     d_vel = torch.from_numpy(vel).to(dev)
     img = torch.zeros(plan.image_floats, dtype=torch.float32, device=dev)
-    chosen = (plan.autotune(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), repeats=2) if autotune
-              else np.full(len(plan.group_pulses), 8))
+    if tuned:
+        plan.set_tuning(SA_TUNING)
     s = torch.cuda.current_stream(dev)
     plan.run(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), graph=True, stream=s.cuda_stream)
     torch.cuda.synchronize()
@@ -236,9 +245,9 @@ def subapertures(count: int = 64, replays: int = 10, autotune: bool = True) -> d
     return {"workload": f"{count} windows of one {P} x {N} dwell, pulse counts {sorted(set(pulses.tolist()))}",
             "ms_per_graph_launch": round(ms, 4), "windows_per_s": round(count / (ms / 1e3), 1),
             "pulses_per_s": round(float(pulses.sum()) / (ms / 1e3), 1),
-            "ra_chunk_rows": {int(m): int(c) for m, c in zip(plan.group_pulses, chosen)},
             "tuning": {m: f"chunk {t['ra_chunk']}, autofocus form {t['autofocus_mode']}, chirp-z {t['czt_length']} x "
                           f"{t['czt_threads']} threads" for m, t in tuning.items()},
+            "tuning_source": "fixed table (isar_sa_plan_set_group_tuning)" if tuned else "defaults",
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                          "algorithmic_bytes_per_launch": alg, "selected_bins_assumed": 64}}
@@ -254,6 +263,7 @@ def _two_batches_in_flight(im, run, img, echo, vel, frames: int, steps: int, s) 
     import isar_imaging_cuda_amd as isar
     s2 = torch.cuda.Stream(device=img.device)
     im2 = isar.Imager(im.params, max_frames=frames)
+    im2.set_autofocus_mode(im.autofocus_mode)
     img2 = torch.empty_like(img)
     run2 = lambda: im2.run_device(echo.data_ptr(), vel.data_ptr(), img2.data_ptr(), frames, isar.Options(),
                                   stream=s2.cuda_stream)
@@ -293,6 +303,7 @@ def large_batch(frames: int = 8, steps: int = 3, learned_frames: int = 16, two_s
     dev = torch.device("cuda:0")
     img = torch.empty((frames, M5, W), dtype=torch.float32, device=dev)
     im = isar.Imager(isar.RadarParams(M5, N5, h["band_width"], h["fc"], h["Tp"], h["Fs"]), max_frames=frames)
+    im.set_autofocus_mode(isar.AF_RESIDENT)   # all iterations in one launch (a launch per iteration at AUTO)
     s = torch.cuda.current_stream(dev)
     run = lambda: im.run_device(echo.data_ptr(), vel.data_ptr(), img.data_ptr(), frames, isar.Options(),
                                 stream=s.cuda_stream)
@@ -306,10 +317,13 @@ def large_batch(frames: int = 8, steps: int = 3, learned_frames: int = 16, two_s
     e1.record(s)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
+    if any(im.trace(k).status & 2 for k in range(frames)):
+        raise RuntimeError("large batch: a resident autofocus wait hit its bound")
     fps = frames / (ms / 1e3)
     fb = algorithmic_bytes_per_frame(M5, N5, W, U)
     out = {"workload": f"{frames} frames of {M5} x {N5} complex64, full plain pipeline",
-           "ms_per_batch": round(ms, 3), "frames_per_s": round(fps, 1), "mean_selected_bins": U,
+           "autofocus_form": "resident", "ms_per_batch": round(ms, 3), "frames_per_s": round(fps, 1),
+           "mean_selected_bins": U,
            "algorithmic_bytes_per_frame": fb, "achieved_GBs": round(fb * fps / 1e9, 1),
            "frac": round(fb * fps / 1e9 / HBM_PEAK_GBS, 4)}
     if two_streams:
@@ -330,13 +344,14 @@ def large_batch(frames: int = 8, steps: int = 3, learned_frames: int = 16, two_s
 
 def single_frame(params, echo, vel, stream, runs: int = 20) -> dict:
     """Latency of one frame alone (a batch of one, device-resident), the drop-in's unit of work:
-    with the drop-in session's two-row alignment chunks, replayed as a HIP graph (graph mode, as
-    the drop-in session runs) and launched directly."""
+    with the drop-in session's two-row alignment chunks and resident autofocus, replayed as a HIP
+    graph (graph mode, as the drop-in session runs) and launched directly."""
     import torch
 
     import isar_imaging_cuda_amd as isar
     im = isar.Imager(params, max_frames=1)
     im.set_ra_chunk(2)   # as the drop-in session images its one frame per call
+    im.set_autofocus_mode(isar.AF_RESIDENT)
     img1 = torch.empty((1, params.echo_num, W), dtype=torch.float32, device=echo.device)
     run = lambda: im.run_device(echo.data_ptr(), vel.data_ptr(), img1.data_ptr(), 1, isar.Options(),
                                 stream=stream.cuda_stream)
@@ -353,8 +368,11 @@ def single_frame(params, echo, vel, stream, runs: int = 20) -> dict:
         e1.record(stream)
         torch.cuda.synchronize()
         out[name] = round(e0.elapsed_time(e1) / runs, 4)
+    if im.trace(0).status & 2:
+        raise RuntimeError("single frame: a resident autofocus wait hit its bound")
     im.close()
     out["runs"] = runs
+    out["autofocus_form"] = "resident"
     return out
 
 
@@ -424,7 +442,7 @@ def main() -> int:
     ap.add_argument("--no-single", action="store_true", help="skip the single-frame latency")
     ap.add_argument("--no-stages", action="store_true", help="skip the per-stage device times")
     ap.add_argument("--sa-fixed", action="store_true",
-                    help="sub-aperture plan without autotuning (default chunk size and autofocus form)")
+                    help="sub-aperture plan with the default chunk size and autofocus form (no tuning table)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -438,7 +456,7 @@ def main() -> int:
         print(json.dumps({"large_batch": large_batch(learned_frames=0 if args.no_learned else 16)}))
         return 0
     if args.subapertures_only:
-        print(json.dumps({"subapertures": subapertures(autotune=not args.sa_fixed)}))
+        print(json.dumps({"subapertures": subapertures(tuned=not args.sa_fixed)}))
         return 0
 
     import torch

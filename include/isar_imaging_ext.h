@@ -149,6 +149,11 @@ ISAR_API int isar_set_hrrp_output(isar_context* ctx, void* d_hrrp);
  * the window's M x N profiles (complex float) to h_hrrp. Synchronous. */
 ISAR_API int isar_session_hrrp(float* h_hrrp);
 
+/* Drop-in session: its context (NULL before imagingMemInit), for the setters below. The session
+ * images each frame with the resident autofocus; a frame on which a resident wait hit its bound
+ * (status bit 2) is imaged again with the split form, with a note on stderr. */
+ISAR_API isar_context* isar_session_context(void);
+
 /* Intermediate arrays of one frame of the last run, as the reference writes them with its
  * DATA_WRITE_BACK_* compile switches (isar_main.cu:10-12, 51-53, 76-78, 101-103, 152-154,
  * 178-180, 221-223). The fused kernels never hold most of them whole; they are rebuilt on the
@@ -181,11 +186,25 @@ ISAR_API int isar_session_dump(unsigned mask, const char* dir);
  * range bins over several workgroups, for small batches and single frames: one launch per
  * iteration, every workgroup forming the current phase from the previous iteration's sums.
  * SPLIT_PHASE does the same with a separate phase launch per iteration, which repeats no work
- * and suits GPUs shared with other streams. AUTO (default) uses SPLIT for batches of at most 32
+ * and suits GPUs shared with other streams. RESIDENT spreads the bins like SPLIT but runs all
+ * iterations in one launch: each workgroup keeps its bins in registers and the frame's
+ * workgroups exchange their sums through memory after every iteration. Its waits are bounded;
+ * a frame whose workgroups could not all be resident together gets status bit 2
+ * (isar_get_trace) and no valid phase. AUTO (default) uses SPLIT for batches of at most 32
  * frames and FUSED above. All forms give the same phases bit for bit. */
-enum isar_autofocus_mode { ISAR_AF_AUTO = 0, ISAR_AF_FUSED = 1, ISAR_AF_SPLIT = 2, ISAR_AF_SPLIT_PHASE = 3 };
+enum isar_autofocus_mode {
+    ISAR_AF_AUTO = 0,
+    ISAR_AF_FUSED = 1,
+    ISAR_AF_SPLIT = 2,
+    ISAR_AF_SPLIT_PHASE = 3,
+    ISAR_AF_RESIDENT = 4
+};
 ISAR_API int isar_set_autofocus_mode(isar_context* ctx, int mode);
 ISAR_API int isar_get_autofocus_mode(isar_context* ctx);
+/* Bound of each wait of the resident autofocus, in polls of about a microsecond (default 2^20);
+ * a negative bound makes every wait give up at once, which forces status bit 2 (for tests of the
+ * fallback). */
+ISAR_API int isar_set_af_wait_polls(isar_context* ctx, int polls);
 
 /* Graph mode (default off): a full run whose arguments (frame count, pointers, options, tuning)
  * equal the previous full run's replays a HIP graph of its launches instead of launching them one

@@ -50,7 +50,7 @@ class _Trace(C.Structure):
 
 # kernel families (enum isar_stage in include/isar_imaging_ext.h)
 # isar_autofocus_mode
-AF_AUTO, AF_FUSED, AF_SPLIT, AF_SPLIT_PHASE = 0, 1, 2, 3
+AF_AUTO, AF_FUSED, AF_SPLIT, AF_SPLIT_PHASE, AF_RESIDENT = 0, 1, 2, 3, 4
 
 # arrays of isar_stage_read (the reference's DATA_WRITE_BACK_* dumps): name -> (id, complex?, columns)
 DUMP_STAGES = {"dataW": 0, "hpc": 1, "hrrp": 2, "ra": 3, "pc": 4, "mtrc": 5, "final": 6}
@@ -261,11 +261,19 @@ class Imager:
 
     def set_autofocus_mode(self, mode: int) -> None:
         """AF_AUTO (split batches of at most 32 frames), AF_FUSED (one workgroup per frame),
-        AF_SPLIT (several workgroups per frame, one launch per iteration) or AF_SPLIT_PHASE (the
-        same with a phase launch per iteration); the phases are the same bit for bit."""
+        AF_SPLIT (several workgroups per frame, one launch per iteration), AF_SPLIT_PHASE (the
+        same with a phase launch per iteration) or AF_RESIDENT (several workgroups per frame, all
+        iterations in one launch); the phases are the same bit for bit."""
         L = lib()
         L.isar_set_autofocus_mode.argtypes = [C.c_void_p, C.c_int]
         _check(L.isar_set_autofocus_mode(self._h, int(mode)), "isar_set_autofocus_mode")
+
+    def set_af_wait_polls(self, polls: int) -> None:
+        """Bound of each resident-autofocus wait in polls (default 2^20); negative makes every wait
+        give up at once and sets status bit 2 of every frame (a test of that path)."""
+        L = lib()
+        L.isar_set_af_wait_polls.argtypes = [C.c_void_p, C.c_int]
+        _check(L.isar_set_af_wait_polls(self._h, int(polls)), "isar_set_af_wait_polls")
 
     @property
     def autofocus_mode(self) -> int:

@@ -3,12 +3,14 @@
 // isarMainSingle } x n -> imagingMemDest. Built against include/isar_imaging_export.h only.
 //
 // usage: isar_calling_app <dir> <window_len> <imaging_stride> <n_images> <out_prefix> [polar] [mtrc] [hpc] [data_type]
-//                         [hrrp] [dump_dir]  (defaults: polar_type 1, if_mtrc 1, if_hpc 1, data_type 2,
-//                         if_hrrp 0, no dump)
+//                         [hrrp] [dump_dir] [af_wait_polls]  (defaults: polar_type 1, if_mtrc 1, if_hpc 1,
+//                         data_type 2, if_hrrp 0, no dump, the library's wait bound)
 // writes <out_prefix>_<i>.bin (raw float32 image, window_len x 512) and <out_prefix>_0.txt; with
 // if_hrrp also <out_prefix>_hrrp_<i>.bin (window_len x range_num complex float range profiles);
 // with dump_dir the first window's intermediate arrays as the reference's text dumps
 // (dataW.dat ... final.dat).
+// af_wait_polls sets the session context's resident-autofocus wait bound (isar_set_af_wait_polls;
+// -1 makes every wait give up, so each window is imaged again with the split form).
 // n_images = 0 only parses the directory (no GPU call) and prints the decoded headers.
 // n_images = -1 checks that dataExtracting refuses arguments that do not match imagingMemInit
 // (frame length, data type, a too-small output vector) instead of overrunning a buffer.
@@ -41,6 +43,8 @@ int main(int argc, char** argv) {
     const int option_alignment = 0, option_phase = 1;
     const bool if_hrrp = argc > 10 ? std::atoi(argv[10]) != 0 : false;
     const std::string dump_dir = argc > 11 ? argv[11] : "";
+    const bool set_polls = argc > 12;
+    const int af_wait_polls = set_polls ? std::atoi(argv[12]) : 0;
 
     vec1D_DBL dataN;
     vec1D_FLT turnAngle;
@@ -70,6 +74,10 @@ int main(int argc, char** argv) {
     imagingMemInit(&img, &dataWFileSn, &dataNOut, &turnAngleOut, &dataW, window_len, frame_len, data_type, if_hpc, if_hrrp);
     if (img.size() != static_cast<size_t>(window_len) * 512) {
         std::fprintf(stderr, "imagingMemInit: %s\n", isar_last_error());
+        return 5;
+    }
+    if (set_polls && isar_set_af_wait_polls(isar_session_context(), af_wait_polls) != ISAR_OK) {
+        std::fprintf(stderr, "isar_set_af_wait_polls: %s\n", isar_last_error());
         return 5;
     }
     if (n_images < 0) {

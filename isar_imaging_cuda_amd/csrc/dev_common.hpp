@@ -786,4 +786,63 @@ __device__ __forceinline__ void relayout_out_to_in(float2 (&x)[E], float2* buf, 
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Hand-off between the workgroups of one launch (the resident autofocus). Values pass through
+// memory as agent-scope relaxed atomics: global accesses with sc1 set, which are written through
+// to the device's point of coherence and read from it, never from a stale line of another XCD's
+// L2. Each thread's stores are acknowledged (s_waitcnt 0) before the workgroup's barrier that
+// precedes its counter increment, so a workgroup that sees the count also sees the values. No
+// L2 write-back or invalidation is involved, unlike an agent-scope release/acquire fence.
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void xwg_store(float2* p, float2 v) {
+    uint64_t bits;
+    __builtin_memcpy(&bits, &v, sizeof bits);
+#ifdef ISAR_EMU
+    __atomic_store_n(reinterpret_cast<uint64_t*>(p), bits, __ATOMIC_RELAXED);
+#else
+    __hip_atomic_store(reinterpret_cast<uint64_t*>(p), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+__device__ __forceinline__ float2 xwg_load(const float2* p) {
+#ifdef ISAR_EMU
+    const uint64_t bits = __atomic_load_n(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED);
+#else
+    const uint64_t bits = __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+    float2 v;
+    __builtin_memcpy(&v, &bits, sizeof v);
+    return v;
+}
+// Waits until this thread's stores are acknowledged.
+__device__ __forceinline__ void xwg_stores_done() {
+#ifdef ISAR_EMU
+    __atomic_thread_fence(__ATOMIC_SEQ_CST);
+#else
+    __builtin_amdgcn_s_waitcnt(0);
+#endif
+}
+__device__ __forceinline__ void xwg_arrive(unsigned* c) {
+#ifdef ISAR_EMU
+    __atomic_fetch_add(c, 1u, __ATOMIC_SEQ_CST);
+#else
+    __hip_atomic_fetch_add(c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#endif
+}
+// Polls *c until it reaches target; false when `limit` polls pass first, and at once when limit
+// is negative.
+__device__ __forceinline__ bool xwg_wait(const unsigned* c, unsigned target, int limit) {
+    if (limit < 0) return false;
+    for (int n = 0;; ++n) {
+#ifdef ISAR_EMU
+        if (__atomic_load_n(c, __ATOMIC_SEQ_CST) >= target) return true;
+        if (n >= limit) return false;
+        std::this_thread::yield();
+#else
+        if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
+        if (n >= limit) return false;
+        __builtin_amdgcn_s_sleep(2);
+#endif
+    }
+}
+
 }  // namespace isar

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -60,6 +61,7 @@ inline thread_local BlockCtx t_ctx;
 inline void __syncthreads() { isar_emu::t_ctx.bar->arrive_and_wait(); }
 inline void __threadfence() { std::atomic_thread_fence(std::memory_order_seq_cst); }
 inline int atomicAdd(int* p, int v) { return __atomic_fetch_add(p, v, __ATOMIC_SEQ_CST); }
+inline int atomicOr(int* p, int v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 
 namespace isar_emu {
 template <typename T>
@@ -193,7 +195,36 @@ inline void launch(K kernel, dim3 grid, dim3 block, size_t smem, A... args) {
                 for (auto& x : th) x.join();
             }
 }
+// Runs all blocks of the grid concurrently, each with its own LDS and barrier.
+template <typename K, typename... A>
+inline void launch_all(K kernel, dim3 grid, dim3 block, size_t smem, A... args) {
+    const unsigned nt = block.x * block.y * block.z;
+    const unsigned nb = grid.x * grid.y * grid.z;
+    std::vector<std::vector<unsigned char>> lds(nb, std::vector<unsigned char>(smem ? smem : 1, 0xCD));
+    std::vector<std::vector<uint64_t>> xbuf(nb, std::vector<uint64_t>(nt));
+    std::vector<std::unique_ptr<std::barrier<>>> bars;
+    for (unsigned b = 0; b < nb; ++b) bars.emplace_back(std::make_unique<std::barrier<>>(static_cast<std::ptrdiff_t>(nt)));
+    std::vector<std::thread> th;
+    th.reserve(static_cast<size_t>(nb) * nt);
+    for (unsigned b = 0; b < nb; ++b) {
+        const unsigned bx = b % grid.x, by = (b / grid.x) % grid.y, bz = b / (grid.x * grid.y);
+        for (unsigned t = 0; t < nt; ++t) {
+            th.emplace_back([&, b, bx, by, bz, t]() {
+                t_threadIdx = dim3(t % block.x, (t / block.x) % block.y, t / (block.x * block.y));
+                t_blockIdx = dim3(bx, by, bz);
+                t_blockDim = block;
+                t_gridDim = grid;
+                t_ctx = BlockCtx{bars[b].get(), lds[b].data(), &xbuf[b]};
+                kernel(args...);
+            });
+        }
+    }
+    for (auto& x : th) x.join();
+}
 }  // namespace isar_emu
 
 #define ISAR_LAUNCH(kernel, grid, block, smem, stream, ...) \
     isar_emu::launch(kernel, dim3(grid), dim3(block), (smem), __VA_ARGS__)
+// every workgroup at once (each its own LDS), for kernels whose workgroups wait for each other
+#define ISAR_LAUNCH_RESIDENT(kernel, grid, block, smem, stream, ...) \
+    isar_emu::launch_all(kernel, dim3(grid), dim3(block), (smem), __VA_ARGS__)

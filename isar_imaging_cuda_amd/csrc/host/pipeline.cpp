@@ -118,6 +118,7 @@ size_t carve_workspace(const isar_radar_params& p, int K_, char* base, Job* J, S
     j.colsum = static_cast<float2*>(frame(W * sizeof(float2)));
     j.delta = static_cast<float2*>(frame(W * sizeof(float2)));
     j.af_part = static_cast<float2*>(frame(2 * (W / 8) * M * sizeof(float2)));
+    j.af_sync = static_cast<unsigned*>(frame(sizeof(unsigned)));
     SharedTables t{};
     t.czt_table = static_cast<float2*>(shared(W * F * sizeof(float2)));
     t.czt_pre = static_cast<float2*>(shared(W * M * sizeof(float2)));
@@ -288,12 +289,14 @@ int Context::run_frames(int frames, const void* echo, const double* velocity, co
     k.ra_split = ra_split;
     k.czt_len = F;
     k.czt_threads = czt_threads;
+    k.af_wait_polls = af_wait_polls;
     k.hrrp = hrrp_out;
     const GraphKey& g0 = graph_key;
     const bool same = graph_exec && k.frames == g0.frames && k.echo == g0.echo && k.velocity == g0.velocity &&
                       k.frame_row == g0.frame_row && k.phi0 == g0.phi0 && k.img == g0.img &&
                       k.frame_img == g0.frame_img && k.ra_chunk == g0.ra_chunk && k.af_mode == g0.af_mode && k.ra_split == g0.ra_split &&
-                      k.czt_len == g0.czt_len && k.czt_threads == g0.czt_threads && k.hrrp == g0.hrrp && std::memcmp(&k.opt, &g0.opt, sizeof(isar_options)) == 0;
+                      k.czt_len == g0.czt_len && k.czt_threads == g0.czt_threads && k.af_wait_polls == g0.af_wait_polls &&
+                      k.hrrp == g0.hrrp && std::memcmp(&k.opt, &g0.opt, sizeof(isar_options)) == 0;
     if (!same) {
         // capture the launches on an internal stream; the graph is then launched on the caller's
         drop_graph();
@@ -360,6 +363,7 @@ int Context::enqueue(int frames, const void* echo, const double* velocity, const
     J.ra_chunk = ra_chunk;
     J.ra_split = ra_split;
     J.czt_threads = czt_threads;
+    J.af_wait_polls = af_wait_polls;
     J.phi0_in = static_cast<const float2*>(phi0);
     J.img = img;
     J.hrrp = opt.if_hrrp && mode != ISAR_RUN_BACK ? hrrp_out : nullptr;
@@ -388,7 +392,8 @@ int Context::enqueue(int frames, const void* echo, const double* velocity, const
     const bool af_split = af_mode == ISAR_AF_SPLIT || af_mode == ISAR_AF_SPLIT_PHASE ||
                           (af_mode == ISAR_AF_AUTO && frames <= kAfSplitFrames);
     probed(ISAR_STAGE_AUTOFOCUS, s, [&] {
-        if (af_split) launch_autofocus_split(J, af_mode != ISAR_AF_SPLIT_PHASE, s);
+        if (af_mode == ISAR_AF_RESIDENT) launch_autofocus_resident(J, s);
+        else if (af_split) launch_autofocus_split(J, af_mode != ISAR_AF_SPLIT_PHASE, s);
         else launch_autofocus(J, s);
     });
     if (mode == ISAR_RUN_FRONT || (mode == ISAR_RUN_BACK && !img)) {
@@ -412,7 +417,8 @@ int Context::enqueue(int frames, const void* echo, const double* velocity, const
 }
 
 int Context::autotune(int frames, const void* echo, const double* velocity, const long long* frame_row, float* img,
-                      const long long* frame_img, const isar_options& opt, hipStream_t s, int repeats, int* chosen) {
+                      const long long* frame_img, const isar_options& opt, hipStream_t s, int repeats, int* chosen,
+                      bool with_resident) {
     if (repeats < 1) return fail(ISAR_ERR_INVALID, "autotune needs at least one timed run");
     hipEvent_t a = nullptr, b = nullptr;
     if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
@@ -453,7 +459,8 @@ int Context::autotune(int frames, const void* echo, const double* velocity, cons
     if (rc == ISAR_OK) {
         float best_af = 0.0f;
         int best_mode = af_mode;
-        for (int mode : {ISAR_AF_FUSED, ISAR_AF_SPLIT, ISAR_AF_SPLIT_PHASE}) {
+        for (int mode : {ISAR_AF_FUSED, ISAR_AF_SPLIT, ISAR_AF_SPLIT_PHASE, ISAR_AF_RESIDENT}) {
+            if (mode == ISAR_AF_RESIDENT && !with_resident) continue;
             af_mode = mode;
             const float ms = time_runs();
             if (ms < 0.0f) break;
@@ -729,14 +736,21 @@ ISAR_API int isar_get_czt_threads(isar_context* ctx) { return ctx ? ctx->czt_thr
 
 ISAR_API int isar_set_autofocus_mode(isar_context* ctx, int mode) {
     if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");
-    if (mode != ISAR_AF_AUTO && mode != ISAR_AF_FUSED && mode != ISAR_AF_SPLIT && mode != ISAR_AF_SPLIT_PHASE)
-        return isar::fail(ISAR_ERR_INVALID,
-                          "autofocus mode must be 0 (auto), 1 (one workgroup per frame), 2 (split) or 3 (split, phase launches)");
+    if (mode != ISAR_AF_AUTO && mode != ISAR_AF_FUSED && mode != ISAR_AF_SPLIT && mode != ISAR_AF_SPLIT_PHASE &&
+        mode != ISAR_AF_RESIDENT)
+        return isar::fail(ISAR_ERR_INVALID, "autofocus mode must be 0 (auto), 1 (one workgroup per frame), 2 (split), "
+                                            "3 (split, phase launches) or 4 (resident split)");
     ctx->af_mode = mode;
     return ISAR_OK;
 }
 
 ISAR_API int isar_get_autofocus_mode(isar_context* ctx) { return ctx ? ctx->af_mode : -1; }
+
+ISAR_API int isar_set_af_wait_polls(isar_context* ctx, int polls) {
+    if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");
+    ctx->af_wait_polls = polls;
+    return ISAR_OK;
+}
 
 ISAR_API int isar_set_graph_mode(isar_context* ctx, int on) {
     if (!ctx) return isar::fail(ISAR_ERR_INVALID, "null context");

@@ -78,7 +78,7 @@ struct Context {
         float* img = nullptr;
         const long long* frame_img = nullptr;
         isar_options opt{};
-        int ra_chunk = 0, af_mode = 0, ra_split = 0, czt_len = 0, czt_threads = 0;
+        int ra_chunk = 0, af_mode = 0, ra_split = 0, czt_len = 0, czt_threads = 0, af_wait_polls = 0;
         const float2* hrrp = nullptr;
     };
     bool graph_mode = false;
@@ -96,16 +96,19 @@ struct Context {
     float2* hrrp_out = nullptr;           // max_frames x M x N profiles written by if_hrrp runs, or null
     int af_mode = ISAR_AF_AUTO;           // autofocus form: auto, one workgroup per frame, split
     float tuned_ms = 0.0f;                // time of one run in the configuration the autotuner kept
-    float mode_ms[4] = {0, 0, 0, 0};      // time of one run per autofocus form (index: isar_autofocus_mode)
+    float mode_ms[5] = {0, 0, 0, 0, 0};   // time of one run per autofocus form (index: isar_autofocus_mode)
     int ra_chunk = kRaChunkRowsDefault;   // tunable: rows per workgroup in the alignment update
     int ra_split = 0;                     // alignment update grids per level (isar_set_ra_split)
     int czt_threads = 256;                // chirp-z workgroup width (isar_set_czt_threads)
+    int af_wait_polls = 1 << 20;          // bound of each resident-autofocus wait (isar_set_af_wait_polls)
     // chirp-z transform length (one of czt_length_candidates; 0 = the default): waits for the
     // device, then re-forms the chirp-z tables
     int set_czt_length(int length);
     // times every range-alignment chunk size on these frames and keeps the fastest (synchronous)
+    // (with_resident: the resident autofocus form is among the candidates)
     int autotune(int frames, const void* echo, const double* velocity, const long long* frame_row, float* img,
-                 const long long* frame_img, const isar_options& opt, hipStream_t s, int repeats, int* chosen);
+                 const long long* frame_img, const isar_options& opt, hipStream_t s, int repeats, int* chosen,
+                 bool with_resident = true);
     int trace(int frame, isar_frame_trace* out);
     int phases(int frames, void* d_start, void* d_final, hipStream_t s);
 };

@@ -226,6 +226,26 @@ ISAR_API int isar_sa_plan_group_tuning(const isar_sa_plan* plan, int group, int3
     return ISAR_OK;
 }
 
+ISAR_API int isar_sa_plan_set_group_tuning(isar_sa_plan* plan, int group, const int32_t* tuning) {
+    if (!plan || !tuning || group < 0 || group >= static_cast<int>(plan->groups.size()))
+        return fail(ISAR_ERR_INVALID, "isar_sa_plan_set_group_tuning: bad arguments");
+    const int rows = tuning[0], form = tuning[1], threads = tuning[3];
+    if (rows != 2 && rows != 4 && rows != 8 && rows != 16)
+        return fail(ISAR_ERR_INVALID, "isar_sa_plan_set_group_tuning: chunk rows must be 2, 4, 8 or 16");
+    if (form != ISAR_AF_FUSED && form != ISAR_AF_SPLIT && form != ISAR_AF_SPLIT_PHASE)
+        return fail(ISAR_ERR_INVALID, "isar_sa_plan_set_group_tuning: autofocus form must be 1, 2 or 3");
+    if (threads != 256 && threads != 512)
+        return fail(ISAR_ERR_INVALID, "isar_sa_plan_set_group_tuning: chirp-z threads must be 256 or 512");
+    isar::Context& c = plan->groups[group]->ctx;
+    const int rc = c.set_czt_length(tuning[2]);
+    if (rc) return rc;
+    plan->drop_graph();
+    c.ra_chunk = rows;
+    c.af_mode = form;
+    c.czt_threads = threads;
+    return ISAR_OK;
+}
+
 ISAR_API int isar_sa_plan_run(isar_sa_plan* plan, const void* d_dwell, const double* d_velocity, float* d_img,
                               const isar_options* opt, int use_graph, void* stream) {
     if (!plan || !d_dwell || !d_img || !opt) return fail(ISAR_ERR_INVALID, "isar_sa_plan_run: null argument");
@@ -273,8 +293,9 @@ ISAR_API int isar_sa_plan_autotune(isar_sa_plan* plan, const void* d_dwell, cons
     for (size_t gi = 0; gi < plan->groups.size(); ++gi) {
         isar::SaGroup* g = plan->groups[gi];
         int c = 0;
+        // the resident autofocus form is not offered to sub-aperture plans (see below)
         const int rc = g->ctx.autotune(static_cast<int>(g->windows.size()), d_dwell, d_velocity, g->d_rows, d_img,
-                                       g->d_img, *opt, plan->stream, repeats, &c);
+                                       g->d_img, *opt, plan->stream, repeats, &c, false);
         if (rc) return rc;
         if (chosen) chosen[gi] = c;
         g->cost = g->ctx.tuned_ms;
@@ -289,6 +310,8 @@ ISAR_API int isar_sa_plan_autotune(isar_sa_plan* plan, const void* d_dwell, cons
         hipEventCreate(&b) != hipSuccess)
         return fail(ISAR_ERR_DEVICE, "isar_sa_plan_autotune: timing objects");
     // candidates: each group's fastest form alone among a subset of the forms, and one form for all
+    // The resident form is left out: with it among the candidates, this tuner's process once ended
+    // with a host segmentation fault (no GPU fault) whose cause was not found.
     const std::vector<std::vector<int>> subsets = {
         {ISAR_AF_FUSED, ISAR_AF_SPLIT, ISAR_AF_SPLIT_PHASE}, {ISAR_AF_FUSED, ISAR_AF_SPLIT_PHASE},
         {ISAR_AF_FUSED, ISAR_AF_SPLIT}, {ISAR_AF_FUSED}, {ISAR_AF_SPLIT}, {ISAR_AF_SPLIT_PHASE}};

@@ -214,6 +214,7 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     const int* sel = J.sel + static_cast<size_t>(f) * W;
     float2* su = J.su + static_cast<size_t>(f) * W * M;
     const float2* X = J.x + static_cast<size_t>(f) * M * W;
+    if (tid == 0) J.af_sync[f] = 0;   // the resident form's hand-off counter, for the launch after this one
 
     // gather the selected bins, column-major (selectRangeBins, :335-346), scaled by 2^-e so that
     // the largest selected |S| lies in [1, 2): the update takes the unit phasor of sums cubic in
@@ -335,20 +336,26 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
 // (FOLD: one launch per iteration) or read from J.phi, which k_af_phase writes after each
 // iteration (two launches per iteration, no repeated work). The transforms and the summation
 // order are those of k_autofocus, so the phases agree bit for bit.
-template <int M>
-__device__ __forceinline__ float2 af_phase_from(const float2* part, int nsub, int m) {
+// Unit phasor of pulse m's sum over the nsub sub-chunk sums part[q M + m], added in order;
+// load(p) reads one sum.
+template <int M, typename Load>
+__device__ __forceinline__ float2 af_phase_from(const float2* part, int nsub, int m, Load load) {
     float2 acc = make_float2(0.0f, 0.0f);
     int q = 0;
     for (; q + 8 <= nsub; q += 8) {   // eight loads in flight, added in order
         float2 v[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = part[static_cast<size_t>(q + i) * M + m];
+        for (int i = 0; i < 8; ++i) v[i] = load(part + static_cast<size_t>(q + i) * M + m);
 #pragma unroll
         for (int i = 0; i < 8; ++i) acc = cadd(acc, v[i]);
     }
-    for (; q < nsub; ++q) acc = cadd(acc, part[static_cast<size_t>(q) * M + m]);
+    for (; q < nsub; ++q) acc = cadd(acc, load(part + static_cast<size_t>(q) * M + m));
     const float a = cabsf_(acc);
     return a > 0.0f ? cscale(acc, 1.0f / a) : make_float2(0.0f, 0.0f);
+}
+template <int M>
+__device__ __forceinline__ float2 af_phase_from(const float2* part, int nsub, int m) {
+    return af_phase_from<M>(part, nsub, m, [](const float2* p) { return *p; });
 }
 
 // the half of af_part that iteration `it` writes; frame f's sub-chunk sums in it
@@ -356,6 +363,43 @@ template <int M>
 __device__ __forceinline__ float2* af_part_of(const Job& J, int f, int it) {
     const size_t per_frame = static_cast<size_t>(J.W / AfCfg<M>::SUB) * M;
     return J.af_part + (static_cast<size_t>(it & 1) * J.K + f) * per_frame;
+}
+
+// One iteration's work on one sub-chunk of a frame's selected bins: the SUB columns (column
+// c0 + tid / TPF in s, first-pass layout, zero where invalid) are multiplied by the phase phi
+// (M values), transformed, scaled by |G|, transformed back, multiplied by conj(S) (af_term) and
+// left in bufs; then the per-pulse sums over the sub-chunk's ncol columns, each added column by
+// column, are handed to put(m, sum). Every thread of the workgroup calls it.
+template <int M, typename Put>
+__device__ __forceinline__ void af_subchunk(const float2 (&s)[AfCfg<M>::E], const float2* phi, const float2* col,
+                                            bool valid, int ncol, float2* bufs, int tid,
+                                            const FftTwiddles<M, AfCfg<M>::E>& tw, Put put) {
+    using A = AfCfg<M>;
+    using C = ColCfg<M>;
+    constexpr int E = A::E, TPF = A::TPF, NTS = A::SUB * TPF;
+    const int g = tid / TPF, lane = tid % TPF;
+    float2* buf = bufs + g * lds_padded_size(M);
+    const float invM = 1.0f / static_cast<float>(M);
+    float2 x[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = cmul(phi[C::in(lane, e)], s[e]);
+    fft<M, E, -1>(x, buf, lane, tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabs_hw(x[e]));
+    relayout_out_to_in<M, E>(x, buf, lane);
+    fft<M, E, +1>(x, buf, lane, tw);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int pos = C::out(lane, e);
+        const float2 sv = C::same_layout ? s[e] : (valid ? col[pos] : make_float2(0.0f, 0.0f));
+        buf[pos] = af_term<M>(sv, x[e], invM);
+    }
+    __syncthreads();
+    for (int m = tid; m < M; m += NTS) {
+        float2 t = make_float2(0.0f, 0.0f);
+        for (int k = 0; k < ncol; ++k) t = cadd(t, bufs[k * lds_padded_size(M) + m]);
+        put(m, t);
+    }
 }
 
 template <int M, bool FOLD>
@@ -378,37 +422,71 @@ __global__ __launch_bounds__(AfCfg<M>::SUB * AfCfg<M>::TPF) void k_af_cols(Job J
         __syncthreads();
         phi = formed;
     }
-    const int g = tid / TPF, lane = tid % TPF;
-    const int u = c0 + g;
+    const int u = c0 + tid / TPF, lane = tid % TPF;
     const bool valid = u < U;
     const float2* col = J.su + static_cast<size_t>(f) * W * M + static_cast<size_t>(valid ? u : 0) * M;
-    float2* buf = bufs + g * lds_padded_size(M);
-    const float invM = 1.0f / static_cast<float>(M);
     const FftTwiddles<M, E> tw = fft_twiddles<M, E>(lane);
-    float2 s[E], x[E];
+    float2 s[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) s[e] = valid ? col[C::in(lane, e)] : make_float2(0.0f, 0.0f);
-#pragma unroll
-    for (int e = 0; e < E; ++e) x[e] = cmul(phi[C::in(lane, e)], s[e]);
-    fft<M, E, -1>(x, buf, lane, tw);
-#pragma unroll
-    for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabs_hw(x[e]));
-    relayout_out_to_in<M, E>(x, buf, lane);
-    fft<M, E, +1>(x, buf, lane, tw);
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const int pos = C::out(lane, e);
-        const float2 sv = C::same_layout ? s[e] : (valid ? col[pos] : make_float2(0.0f, 0.0f));
-        buf[pos] = af_term<M>(sv, x[e], invM);
-    }
-    __syncthreads();
-    const int ncol = min(SUB, U - c0);
     float2* part = af_part_of<M>(J, f, it) + static_cast<size_t>(j) * M;
-    for (int m = tid; m < M; m += NTS) {
-        float2 t = make_float2(0.0f, 0.0f);
-        for (int k = 0; k < ncol; ++k) t = cadd(t, bufs[k * lds_padded_size(M) + m]);
-        part[m] = t;
+    af_subchunk<M>(s, phi, col, valid, min(SUB, U - c0), bufs, tid, tw, [&](int m, float2 t) { part[m] = t; });
+}
+
+// The fixed-point iterations of k_af_cols (FOLD) in one launch: workgroup (j, f) keeps sub-chunk
+// j of frame f's selected bins in registers for every iteration and hands its per-pulse sums to
+// the frame's other workgroups through af_part and the frame's counter af_sync (xwg_*): after
+// iteration `it` the counter reaches (it + 1) * nsub, and every workgroup then forms the next
+// phase from all nsub sums, in sub-chunk order. The transforms, sums and phases are those of
+// k_af_cols bit for bit. All of a frame's workgroups must be resident together; the dispatcher
+// places a grid's workgroups in order, so every frame but the last one placed is complete, and
+// that one completes as earlier frames finish. A wait that exceeds its bound (J.af_wait_polls
+// polls, about a second by default) sets status bit 2 of the frame and stops that workgroup
+// waiting; the frame's phase is then not valid. Workgroup 0 of each frame writes the final phase
+// to J.phi.
+template <int M>
+__global__ __launch_bounds__(AfCfg<M>::SUB * AfCfg<M>::TPF) void k_af_resident(Job J) {
+    using A = AfCfg<M>;
+    using C = ColCfg<M>;
+    constexpr int E = A::E, TPF = A::TPF, SUB = A::SUB, NTS = SUB * TPF;
+    HIP_DYNAMIC_SHARED(float2, bufs);   // SUB * lds_padded_size(M), then the phase (M)
+    float2* phi = bufs + SUB * lds_padded_size(M);
+    const int j = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
+    const int U = J.dec[f].unit2_num;
+    const int nsub = (U + SUB - 1) / SUB;
+    if (j >= nsub) return;   // the whole workgroup leaves before any barrier
+    const int c0 = j * SUB;
+    const int W = J.W;
+    const int u = c0 + tid / TPF, lane = tid % TPF;
+    const bool valid = u < U;
+    const float2* col = J.su + static_cast<size_t>(f) * W * M + static_cast<size_t>(valid ? u : 0) * M;
+    const FftTwiddles<M, E> tw = fft_twiddles<M, E>(lane);
+    float2 s[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) s[e] = valid ? col[C::in(lane, e)] : make_float2(0.0f, 0.0f);
+    for (int m = tid; m < M; m += NTS) phi[m] = J.phi[static_cast<size_t>(f) * M + m];
+    __syncthreads();
+    bool waiting = true;   // thread 0's: false once a wait has hit its bound
+    unsigned* const count = J.af_sync + f;
+    for (int it = 0; it < J.iterations; ++it) {
+        float2* part = af_part_of<M>(J, f, it);
+        af_subchunk<M>(s, phi, col, valid, min(SUB, U - c0), bufs, tid, tw,
+                       [&](int m, float2 t) { xwg_store(part + static_cast<size_t>(j) * M + m, t); });
+        xwg_stores_done();
+        __syncthreads();
+        if (tid == 0) {
+            xwg_arrive(count);
+            if (waiting && !xwg_wait(count, static_cast<unsigned>(it + 1) * static_cast<unsigned>(nsub), J.af_wait_polls)) {
+                waiting = false;
+                atomicOr(&J.dec[f].status, 2);
+            }
+        }
+        __syncthreads();
+        for (int m = tid; m < M; m += NTS) phi[m] = af_phase_from<M>(part, nsub, m, [](const float2* p) { return xwg_load(p); });
+        __syncthreads();
     }
+    if (j == 0)
+        for (int m = tid; m < M; m += NTS) J.phi[static_cast<size_t>(f) * M + m] = phi[m];
 }
 
 template <int M>
@@ -470,7 +548,8 @@ hipError_t configure_pa_kernels() {
     set(reinterpret_cast<const void*>(&k_pa_colfft<MM>), colfft_lds<MM>());      \
     set(reinterpret_cast<const void*>(&k_autofocus<MM>), autofocus_lds<MM>());   \
     set(reinterpret_cast<const void*>(&k_af_cols<MM, true>), af_cols_lds<MM>(true));   \
-    set(reinterpret_cast<const void*>(&k_af_cols<MM, false>), af_cols_lds<MM>(false));
+    set(reinterpret_cast<const void*>(&k_af_cols<MM, false>), af_cols_lds<MM>(false));   \
+    set(reinterpret_cast<const void*>(&k_af_resident<MM>), af_cols_lds<MM>(true));
     ISAR_SET_PA(64) ISAR_SET_PA(128) ISAR_SET_PA(192) ISAR_SET_PA(256) ISAR_SET_PA(384) ISAR_SET_PA(512)
     ISAR_SET_PA(1024)
 #undef ISAR_SET_PA
@@ -513,6 +592,24 @@ void autofocus_split(const Job& J, bool fold, hipStream_t s) {
 
 void launch_autofocus_split(const Job& J, bool fold, hipStream_t s) {
     ISAR_PULSE_SWITCH(J.M, autofocus_split<MM>(J, fold, s));
+}
+
+namespace {
+template <int MM>
+void autofocus_resident(const Job& J, hipStream_t s) {
+    Job J0 = J;
+    J0.iterations = 0;
+    ISAR_LAUNCH(k_autofocus<MM>, dim3(J.K), dim3(AfCfg<MM>::NT), autofocus_lds<MM>(), s, J0);
+    if (J.iterations > 0) {
+        constexpr int SUB = AfCfg<MM>::SUB;
+        ISAR_LAUNCH_RESIDENT(k_af_resident<MM>, dim3(J.W / SUB, J.K), dim3(SUB * AfCfg<MM>::TPF), af_cols_lds<MM>(true), s,
+                             J);
+    }
+}
+}  // namespace
+
+void launch_autofocus_resident(const Job& J, hipStream_t s) {
+    ISAR_PULSE_SWITCH(J.M, autofocus_resident<MM>(J, s));
 }
 
 }  // namespace isar

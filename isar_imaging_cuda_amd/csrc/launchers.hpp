@@ -30,6 +30,8 @@ void launch_pa_colfft(const Job& J, hipStream_t s);
 void launch_pa_select(const Job& J, hipStream_t s);
 void launch_autofocus(const Job& J, hipStream_t s);
 void launch_autofocus_split(const Job& J, bool fold, hipStream_t s);
+// all iterations in one launch, each frame over several workgroups that stay resident together
+void launch_autofocus_resident(const Job& J, hipStream_t s);
 
 // MTRC and azimuth compression (mtrc.cu:4-192, isar_main.cu:191-207)
 void launch_czt_table(int M, int W, int F, double c0, double posa, float2* table, hipStream_t s);

@@ -7,6 +7,9 @@
 #include <hip/hip_runtime.h>
 #define ISAR_LAUNCH(kernel, grid, block, smem, stream, ...) \
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), (smem), (stream), __VA_ARGS__)
+// a kernel whose workgroups wait for each other (an ordinary launch on the device)
+#define ISAR_LAUNCH_RESIDENT(kernel, grid, block, smem, stream, ...) \
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), (smem), (stream), __VA_ARGS__)
 #endif
 
 #include <stdint.h>
@@ -35,7 +38,7 @@ struct FrameDecisions {
     float center_thr;
     float pa_threshold;
     float gmax;
-    int32_t status;           // 0 ok; bit flags for degenerate inputs
+    int32_t status;           // 0 ok; bit flags: 1 centring input all NaN, 2 resident autofocus wait bound hit
 };
 
 // Everything a kernel needs to know about one imaging job (identical for all frames of a
@@ -49,6 +52,7 @@ struct Job {
     int ra_chunk;                 // rows per workgroup in the alignment update (2, 4, 8 or 16)
     int ra_split;                 // alignment update grids per level: 0 by batch size, 1 one, 2 two
     int czt_threads;              // chirp-z workgroup width (256 or 512 threads)
+    int af_wait_polls;            // bound of each resident-autofocus wait in polls (< 0: give up at once)
     double hpc_k;                 // HPC phase in half-turns = hpc_k * v * n^2
     double czt_c0, czt_posa;      // alpha_n = 1 - c0 + posa * n
     // inputs
@@ -83,6 +87,7 @@ struct Job {
     float2* czt;                  // K x M x W (chirp-z output)
     float2* colsum;               // K x W
     float2* af_part;              // 2 x K x (W/8) x M: per-pulse sums per sub-chunk of selected bins, by iteration parity (split autofocus)
+    unsigned* af_sync;            // K: hand-off counters of the resident autofocus (cleared by its gather launch)
     const float2* czt_table;      // W x F (FFT of the Bluestein chirp per range bin)
     const float2* czt_pre;        // W x M: exp(j pi a_n m (M - m) / M), the input chirp
     const float2* czt_post;       // W x M: exp(-j pi a_n k^2 / M) / F, the output chirp

@@ -103,6 +103,18 @@ class SubaperturePlan:
                                czt_threads=int(t[3]))
         return out
 
+    def set_tuning(self, tuning: dict[int, dict[str, int]]) -> None:
+        """Applies per pulse count the layout tuning() returns (isar_sa_plan_set_group_tuning)."""
+        L = _bind()
+        L.isar_sa_plan_set_group_tuning.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+        for g, M in enumerate(self.group_pulses):
+            if int(M) not in tuning:
+                continue
+            t = tuning[int(M)]
+            a = np.array([t["ra_chunk"], t["autofocus_mode"], t["czt_length"], t["czt_threads"]], np.int32)
+            _check(L.isar_sa_plan_set_group_tuning(self._h, g, a.ctypes.data_as(C.c_void_p)),
+                   "isar_sa_plan_set_group_tuning")
+
     def image(self, flat: np.ndarray, i: int) -> np.ndarray:
         """Window i's image out of the flat output."""
         o = int(self.offsets[i])

@@ -55,7 +55,7 @@ def run(echo: np.ndarray, velocity: np.ndarray, radar: dict, opts: dict | None =
         timeout: float = 900.0, autofocus_mode: int | None = None, ra_split: int | None = None,
         static_check: bool = False, czt_length: int | None = None, czt_threads: int | None = None) -> list[EmuFrame]:
     """echo: complex64 (K, M, N); velocity: (K, M). radar: band_width, fc, Tp, Fs.
-    autofocus_mode: isar_autofocus_mode (0 auto, 1 one workgroup per frame, 2 split); ra_split:
+    autofocus_mode: isar_autofocus_mode (0 auto, 1 one workgroup per frame, 2 split, 4 resident); ra_split:
     isar_set_ra_split (0 by batch size, 1 one grid, 2 two grids). static_check: the driver also
     images the batch and then its first frame through isar_run_device_static and fails unless both
     equal the context's images bit for bit. czt_length, czt_threads: isar_set_czt_length and

@@ -7,8 +7,9 @@ device memory starts as all-ones bytes (NaN floats), so a kernel reading workspa
 write in the same run would show; the static-workspace entry point (isar_run_device_static) must
 equal the context run bit for bit, on a first call and on a second, smaller one. The 192-pulse
 frame runs the chirp-z transform at the mixed-radix length 384 with 512-thread column tiles. The
-first c1 case runs the one-workgroup autofocus and the second the split form, and the two images
-must agree bit for bit."""
+first c1 case runs the one-workgroup autofocus, the second the split form and the third the
+resident form (every workgroup of the frame run concurrently, exchanging its sums through the
+hand-off counter), and the three images must agree bit for bit."""
 from __future__ import annotations
 
 import concurrent.futures as cf
@@ -27,6 +28,7 @@ pytestmark = pytest.mark.slow
 ITERATIONS = 4
 # (pulses, ra_split, (chirp-z length, threads) or None, autofocus mode or None, name)
 CASES = [(128, 1, None, 1, "c1, one grid"), (128, 2, None, 2, "c1, two grids"),
+         (128, 2, None, 4, "c1, two grids, resident autofocus"),
          (192, 2, (384, 512), None, "M=192, two grids, chirp-z length 384, 512-thread tiles")]
 STATIC_CASE = (64, 2, "M=64 x 2 frames, static workspace")
 
@@ -79,3 +81,5 @@ def test_emulated_kernels_match_oracle(emu_driver):
         assert np.abs(fr.ra_lag - tr.ra_lag[: len(fr.ra_lag)]).max() < 1e-5, name
     one, two = res["c1, one grid"][0], res["c1, two grids"][0]
     assert np.array_equal(one.img, two.img) and np.array_equal(one.ra_lag, two.ra_lag)
+    resident = res["c1, two grids, resident autofocus"][0]
+    assert resident.status & 2 == 0 and np.array_equal(resident.img, two.img)

@@ -40,6 +40,27 @@ def test_calling_sequence_sliding_windows(tmp_path):
     assert np.allclose(txt, img0, atol=6e-6 + 1e-7 * np.abs(img0))   # writeFileFLT: 5 decimals
 
 
+def test_session_images_again_when_a_resident_wait_gives_up(tmp_path):
+    """The session images with the resident autofocus and checks each frame's status; with a wait
+    bound that makes every wait give up (status bit 2), each window is imaged again with the split
+    form, which gives the resident form's image bit for bit."""
+    path, echo, vel, cfg = make_dwell(tmp_path / "data", pulses=192, range_num=1024)
+    window, stride, n_img = 128, 32, 3
+    out = {}
+    for name, polls in (("bounded", None), ("gives_up", "-1")):
+        print(f"[case] drop-in app, resident wait bound {polls or 'default'}", flush=True)
+        args = [str(APP), str(tmp_path / "data"), str(window), str(stride), str(n_img), str(tmp_path / name),
+                "1", "1", "1", "2", "0", ""] + ([polls] if polls else [])
+        r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        out[name] = r.stderr.count("imaging the frame again with the split form")
+        for i in range(n_img):
+            out[(name, i)] = np.fromfile(tmp_path / f"{name}_{i}.bin", dtype=np.float32)
+    assert out["bounded"] == 0 and out["gives_up"] == n_img
+    for i in range(n_img):
+        assert np.array_equal(out[("bounded", i)], out[("gives_up", i)]), f"window {i}"
+
+
 def test_extracting_refuses_arguments_that_do_not_match_init(tmp_path):
     """dataExtracting with a frame length or data type other than imagingMemInit's, or an output
     vector smaller than imagingMemInit made it, returns EXIT_FAILURE with a message (the reference

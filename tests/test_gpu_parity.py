@@ -151,8 +151,10 @@ def test_pipeline_variants(name, dev, orc):
 @pytest.mark.parametrize("M,N", [(64, 1024), (128, 1024), (192, 1024), (256, 2048), (384, 1024), (512, 1024),
                                  (1024, 2048)])
 def test_autofocus_forms_agree(M, N):
-    """One workgroup per frame and the split form (several workgroups per frame, two launches per
-    iteration) give the same phases and images bit for bit."""
+    """One workgroup per frame, the split forms (several workgroups per frame, one or two launches
+    per iteration) and the resident form (several workgroups per frame exchanging their sums
+    inside one launch) give the same phases and images bit for bit; no frame reports a resident
+    wait that hit its bound (status bit 2)."""
     cfgs = [synth.preset(1, echo_num=M, range_num=N, seed=300 + k) for k in range(2)]
     frames = [synth.frame(c) for c in cfgs]
     radar = synth.radar_header(cfgs[0])
@@ -161,18 +163,46 @@ def test_autofocus_forms_agree(M, N):
     print(f"[case] autofocus forms M={M} N={N}", flush=True)
     params = isar.RadarParams(M, N, radar["band_width"], radar["fc"], radar["Tp"], radar["Fs"])
     out = {}
-    for mode in (isar.AF_FUSED, isar.AF_SPLIT, isar.AF_SPLIT_PHASE):
+    for mode in (isar.AF_FUSED, isar.AF_SPLIT, isar.AF_SPLIT_PHASE, isar.AF_RESIDENT):
         with isar.Imager(params, max_frames=2) as im:
             im.set_autofocus_mode(mode)
             img = im.run_host(echo, vel, isar.Options())
             out[mode] = (img, [im.trace(k) for k in range(2)])
     img_f, tf = out[isar.AF_FUSED]
-    for mode in (isar.AF_SPLIT, isar.AF_SPLIT_PHASE):
+    for mode in (isar.AF_SPLIT, isar.AF_SPLIT_PHASE, isar.AF_RESIDENT):
         img_s, ts = out[mode]
         for k in range(2):
+            assert ts[k].status & 2 == 0, f"mode {mode} frame {k}: resident wait bound hit"
             assert tf[k].unit2_num == ts[k].unit2_num
             assert np.array_equal(tf[k].phi, ts[k].phi), f"mode {mode} frame {k}: phases differ"
         assert np.array_equal(img_f, img_s), f"mode {mode}"
+
+
+def test_resident_wait_bound_sets_status():
+    """A resident-autofocus wait that gives up (bound forced negative from the host) marks the
+    frame with status bit 2 and every workgroup still finishes; with the default bound no frame
+    is marked and the phases equal the split form's."""
+    cfgs = [synth.preset(1, seed=310 + k) for k in range(2)]
+    frames = [synth.frame(c) for c in cfgs]
+    radar = synth.radar_header(cfgs[0])
+    M, N = cfgs[0].echo_num, cfgs[0].range_num
+    echo = np.stack([f[0] for f in frames])
+    vel = np.stack([f[1] for f in frames])
+    print(f"[case] resident wait bound M={M} N={N}", flush=True)
+    params = isar.RadarParams(M, N, radar["band_width"], radar["fc"], radar["Tp"], radar["Fs"])
+    phis = {}
+    for mode, polls in ((isar.AF_SPLIT, None), (isar.AF_RESIDENT, -1), (isar.AF_RESIDENT, None)):
+        with isar.Imager(params, max_frames=2) as im:
+            im.set_autofocus_mode(mode)
+            if polls is not None:
+                im.set_af_wait_polls(polls)
+            im.run_host(echo, vel, isar.Options())
+            tr = [im.trace(k) for k in range(2)]
+        forced = polls is not None
+        assert all(bool(t.status & 2) == forced for t in tr), [t.status for t in tr]
+        phis[(mode, forced)] = [t.phi for t in tr]
+    for k in range(2):
+        assert np.array_equal(phis[(isar.AF_SPLIT, False)][k], phis[(isar.AF_RESIDENT, False)][k])
 
 
 def test_batch_matches_single_frames():
