@@ -49,9 +49,9 @@ ISAR_API int isar_sa_plan_groups(const isar_sa_plan* plan, int32_t* pulses, int3
  * [1] autofocus form (isar_autofocus_mode), [2] chirp-z length, [3] chirp-z workgroup threads. */
 ISAR_API int isar_sa_plan_group_tuning(const isar_sa_plan* plan, int group, int32_t* tuning);
 /* Sets the tuning of group g in the same layout (for example one found by isar_sa_plan_autotune
- * earlier): chunk rows 2, 4, 8 or 16; autofocus form 1, 2 or 3; a chirp-z length that
- * isar_czt_lengths lists for the group's pulse count; 256 or 512 threads. Waits for the device
- * when the chirp-z length changes. Invalidates a captured graph. */
+ * earlier): chunk rows 2, 4, 8 or 16; autofocus form 1 to 4 (isar_autofocus_mode); a chirp-z
+ * length that isar_czt_lengths lists for the group's pulse count; 256 or 512 threads. Waits for
+ * the device when the chirp-z length changes. Invalidates a captured graph. */
 ISAR_API int isar_sa_plan_set_group_tuning(isar_sa_plan* plan, int group, const int32_t* tuning);
 
 #ifdef __cplusplus

@@ -232,8 +232,8 @@ ISAR_API int isar_sa_plan_set_group_tuning(isar_sa_plan* plan, int group, const 
     const int rows = tuning[0], form = tuning[1], threads = tuning[3];
     if (rows != 2 && rows != 4 && rows != 8 && rows != 16)
         return fail(ISAR_ERR_INVALID, "isar_sa_plan_set_group_tuning: chunk rows must be 2, 4, 8 or 16");
-    if (form != ISAR_AF_FUSED && form != ISAR_AF_SPLIT && form != ISAR_AF_SPLIT_PHASE)
-        return fail(ISAR_ERR_INVALID, "isar_sa_plan_set_group_tuning: autofocus form must be 1, 2 or 3");
+    if (form != ISAR_AF_FUSED && form != ISAR_AF_SPLIT && form != ISAR_AF_SPLIT_PHASE && form != ISAR_AF_RESIDENT)
+        return fail(ISAR_ERR_INVALID, "isar_sa_plan_set_group_tuning: autofocus form must be 1, 2, 3 or 4");
     if (threads != 256 && threads != 512)
         return fail(ISAR_ERR_INVALID, "isar_sa_plan_set_group_tuning: chirp-z threads must be 256 or 512");
     isar::Context& c = plan->groups[group]->ctx;

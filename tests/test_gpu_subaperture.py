@@ -118,3 +118,24 @@ def test_windows_match_oracle():
         assert abs(e - eo) <= ENTROPY_REL * abs(eo), f"window {i} (M={M}) entropy"
         worst[M] = max(worst.get(M, 0.0), rel)
     print("worst image rel L2 per pulse count:", {k: f"{v:.1e}" for k, v in sorted(worst.items())})
+
+
+def test_resident_groups_match_split_groups():
+    """The same plan with every group's autofocus in the split form and then in the resident form
+    (all iterations of a window in one launch, the groups' launches on concurrent graph branches):
+    graph replays give the same images bit for bit."""
+    import torch
+    plan, echo, vel, h, d_echo, d_vel, img = _setup(count=16, seed=3)
+    s = torch.cuda.current_stream().cuda_stream
+    out = {}
+    for form in (isar.AF_SPLIT, isar.AF_RESIDENT):
+        plan.set_tuning({M: dict(t, autofocus_mode=form) for M, t in TUNING.items()})
+        print(f"[case] 16 windows, autofocus form {form} in every group", flush=True)
+        for _ in range(2):
+            img.zero_()
+            plan.run(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), graph=True, stream=s)
+            torch.cuda.synchronize()
+            _ = (d_echo[:1024] * 2).sum().item()    # eager work between replays
+        out[form] = img.cpu().numpy()
+    assert np.isfinite(out[isar.AF_SPLIT]).all()
+    assert np.array_equal(out[isar.AF_SPLIT], out[isar.AF_RESIDENT])
