@@ -143,7 +143,9 @@ int Context::probe_enable(int stage) {
     probe_ms = 0.0;
     probe_launches = 0;
     if (stage >= 0 && probe_events.empty()) {
-        probe_events.resize(2 * 512);
+        // 128 launches between collections (a run probes at most one launch per merge level of
+        // a stage, and run_frames collects before the pool can run out)
+        probe_events.resize(2 * 128);
         for (auto& e : probe_events)
             if (hipEventCreate(&e) != hipSuccess) return fail(ISAR_ERR_DEVICE, "hipEventCreate failed");
     }
