@@ -226,6 +226,7 @@ def subapertures(count: int = 64, replays: int = 10, tuned: bool = True) -> dict
     img = torch.zeros(plan.image_floats, dtype=torch.float32, device=dev)
     if tuned:
         plan.set_tuning(SA_TUNING)
+        plan.calibrate(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), repeats=2)   # branches by measured times
     s = torch.cuda.current_stream(dev)
     plan.run(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), graph=True, stream=s.cuda_stream)
     torch.cuda.synchronize()
@@ -237,6 +238,7 @@ def subapertures(count: int = 64, replays: int = 10, tuned: bool = True) -> dict
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / replays
     tuning = plan.tuning()
+    plan_groups, lanes = plan.group_pulses.tolist(), plan.lanes()
     plan.close()
     # minimum-pass bytes of every window of the launch; U = 64 selected bins per window (the 8MU
     # term is at most 0.13 % of a window's bytes at 512 x 4096, so the assumption barely matters)
@@ -247,7 +249,9 @@ def subapertures(count: int = 64, replays: int = 10, tuned: bool = True) -> dict
             "pulses_per_s": round(float(pulses.sum()) / (ms / 1e3), 1),
             "tuning": {m: f"chunk {t['ra_chunk']}, autofocus form {t['autofocus_mode']}, chirp-z {t['czt_length']} x "
                           f"{t['czt_threads']} threads" for m, t in tuning.items()},
-            "tuning_source": "fixed table (isar_sa_plan_set_group_tuning)" if tuned else "defaults",
+            "tuning_source": ("fixed table (isar_sa_plan_set_group_tuning), branches by measured group times "
+                              "(isar_sa_plan_calibrate)" if tuned else "defaults"),
+            "branches": {int(m): b for m, b in zip(plan_groups, lanes)},
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
                          "algorithmic_bytes_per_launch": alg, "selected_bins_assumed": 64}}
@@ -437,6 +441,7 @@ def main() -> int:
     ap.add_argument("--no-subapertures", action="store_true", help="skip the sub-aperture plan measurement")
     ap.add_argument("--no-large", action="store_true", help="skip the 8 x 1024 x 16384 batch")
     ap.add_argument("--large-only", action="store_true", help="only the 8 x 1024 x 16384 batch (for profiling)")
+    ap.add_argument("--single-only", action="store_true", help="only the one-frame latency (for profiling)")
     ap.add_argument("--subapertures-only", action="store_true", help="only the sub-aperture plan (for profiling)")
     ap.add_argument("--no-transfer", action="store_true", help="skip the host-transfer (PCIe) timing")
     ap.add_argument("--no-single", action="store_true", help="skip the single-frame latency")
@@ -457,6 +462,20 @@ def main() -> int:
         return 0
     if args.subapertures_only:
         print(json.dumps({"subapertures": subapertures(tuned=not args.sa_fixed)}))
+        return 0
+    if args.single_only:
+        import torch
+
+        import isar_imaging_cuda_amd as isar
+        from isar_imaging_cuda_amd import synth
+        dev = torch.device("cuda:0")
+        cfg = synth.preset(2, seed=0xBEEF)
+        e1, v1 = synth.frame(cfg)
+        h = synth.radar_header(cfg)
+        params = isar.RadarParams(cfg.echo_num, cfg.range_num, h["band_width"], h["fc"], h["Tp"], h["Fs"])
+        echo1 = torch.from_numpy(e1[None].view(np.float32)).to(dev)
+        vel1 = torch.from_numpy(v1[None]).to(dev)
+        print(json.dumps({"single_frame": single_frame(params, echo1, vel1, torch.cuda.current_stream(dev))}))
         return 0
 
     import torch

@@ -53,6 +53,13 @@ ISAR_API int isar_sa_plan_group_tuning(const isar_sa_plan* plan, int group, int3
  * length that isar_czt_lengths lists for the group's pulse count; 256 or 512 threads. Waits for
  * the device when the chirp-z length changes. Invalidates a captured graph. */
 ISAR_API int isar_sa_plan_set_group_tuning(isar_sa_plan* plan, int group, const int32_t* tuning);
+/* Times each group alone with its current tuning (one warm-up and `repeats` runs, eager, on an
+ * internal stream) and places the groups on the graph's branches by those times, longest first
+ * onto the least loaded branch. Synchronous. Invalidates a captured graph. */
+ISAR_API int isar_sa_plan_calibrate(isar_sa_plan* plan, const void* d_dwell, const double* d_velocity, float* d_img,
+                                    const isar_options* opt, int repeats);
+/* Graph branch that group g is placed on, or -1 for a bad argument. */
+ISAR_API int isar_sa_plan_group_lane(const isar_sa_plan* plan, int group);
 
 #ifdef __cplusplus
 }

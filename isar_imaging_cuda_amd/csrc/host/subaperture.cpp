@@ -20,7 +20,7 @@ struct SaGroup {
     Context ctx;
     long long* d_rows = nullptr;        // first dwell row of each window
     long long* d_img = nullptr;         // image offset of each window
-    double cost = 0.0;                  // expected time: a model at creation, the autotuner's measure after
+    double cost = 0.0;                  // expected time: a model at creation, measured by calibration or the autotuner
     int lane = 0;                       // branch stream the group is captured on
 };
 
@@ -244,6 +244,43 @@ ISAR_API int isar_sa_plan_set_group_tuning(isar_sa_plan* plan, int group, const 
     c.af_mode = form;
     c.czt_threads = threads;
     return ISAR_OK;
+}
+
+ISAR_API int isar_sa_plan_group_lane(const isar_sa_plan* plan, int group) {
+    if (!plan || group < 0 || group >= static_cast<int>(plan->groups.size())) return -1;
+    return plan->groups[group]->lane;
+}
+
+ISAR_API int isar_sa_plan_calibrate(isar_sa_plan* plan, const void* d_dwell, const double* d_velocity, float* d_img,
+                                    const isar_options* opt, int repeats) {
+    if (!plan || !d_dwell || !d_img || !opt || repeats < 1)
+        return fail(ISAR_ERR_INVALID, "isar_sa_plan_calibrate: bad arguments");
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+        if (a) (void)hipEventDestroy(a);
+        return fail(ISAR_ERR_DEVICE, "isar_sa_plan_calibrate: events");
+    }
+    plan->drop_graph();
+    int rc = ISAR_OK;
+    for (isar::SaGroup* g : plan->groups) {
+        const int frames = static_cast<int>(g->windows.size());
+        auto one = [&]() {
+            return g->ctx.run_frames(frames, d_dwell, d_velocity, g->d_rows, nullptr, d_img, g->d_img, *opt, plan->stream);
+        };
+        rc = one();   // warm-up
+        if (rc == ISAR_OK && hipEventRecord(a, plan->stream) != hipSuccess) rc = fail(ISAR_ERR_DEVICE, "calibrate: record");
+        for (int r = 0; r < repeats && rc == ISAR_OK; ++r) rc = one();
+        if (rc == ISAR_OK && (hipEventRecord(b, plan->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess))
+            rc = fail(ISAR_ERR_DEVICE, "calibrate: timing");
+        if (rc) break;
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, a, b);
+        g->cost = ms / static_cast<float>(repeats);
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc == ISAR_OK) plan->assign_lanes();
+    return rc;
 }
 
 ISAR_API int isar_sa_plan_run(isar_sa_plan* plan, const void* d_dwell, const double* d_velocity, float* d_img,

@@ -115,6 +115,22 @@ class SubaperturePlan:
             _check(L.isar_sa_plan_set_group_tuning(self._h, g, a.ctypes.data_as(C.c_void_p)),
                    "isar_sa_plan_set_group_tuning")
 
+    def calibrate(self, dwell_ptr: int, vel_ptr: int, img_ptr: int, options: Options | None = None,
+                  repeats: int = 2) -> None:
+        """Times each group alone with its current tuning and places the groups on the graph's
+        branches by those times (isar_sa_plan_calibrate)."""
+        o = (options or Options()).c()
+        L = _bind()
+        L.isar_sa_plan_calibrate.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        _check(L.isar_sa_plan_calibrate(self._h, C.c_void_p(dwell_ptr), C.c_void_p(vel_ptr or 0), C.c_void_p(img_ptr),
+                                        C.byref(o), int(repeats)), "isar_sa_plan_calibrate")
+
+    def lanes(self) -> list[int]:
+        """Graph branch of each group (groups in increasing pulse count)."""
+        L = _bind()
+        L.isar_sa_plan_group_lane.argtypes = [C.c_void_p, C.c_int]
+        return [int(L.isar_sa_plan_group_lane(self._h, g)) for g in range(self.n_groups)]
+
     def image(self, flat: np.ndarray, i: int) -> np.ndarray:
         """Window i's image out of the flat output."""
         o = int(self.offsets[i])

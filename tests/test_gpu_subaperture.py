@@ -139,3 +139,24 @@ def test_resident_groups_match_split_groups():
         out[form] = img.cpu().numpy()
     assert np.isfinite(out[isar.AF_SPLIT]).all()
     assert np.array_equal(out[isar.AF_SPLIT], out[isar.AF_RESIDENT])
+
+
+def test_calibration_places_groups_without_changing_images():
+    """isar_sa_plan_calibrate times each group alone and re-places the groups on the graph's
+    branches; the images of a graph replay are the same bit for bit."""
+    import torch
+    plan, echo, vel, h, d_echo, d_vel, img = _setup(count=16, seed=3)
+    s = torch.cuda.current_stream().cuda_stream
+    plan.set_tuning(TUNING)
+    out = []
+    for calibrate in (False, True):
+        if calibrate:
+            plan.calibrate(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), repeats=2)
+        lanes = plan.lanes()
+        print(f"[case] 16 windows, calibrated {calibrate}, branches {lanes}", flush=True)
+        assert all(0 <= b < len(plan.group_pulses) for b in lanes)
+        img.zero_()
+        plan.run(d_echo.data_ptr(), d_vel.data_ptr(), img.data_ptr(), graph=True, stream=s)
+        torch.cuda.synchronize()
+        out.append(img.cpu().numpy())
+    assert np.array_equal(out[0], out[1])
