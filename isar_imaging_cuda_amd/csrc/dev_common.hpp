@@ -788,30 +788,32 @@ __device__ __forceinline__ void relayout_out_to_in(float2 (&x)[E], float2* buf, 
 
 // ------------------------------------------------------------------------------------------
 // Hand-off between the workgroups of one launch (the resident autofocus). Values pass through
-// memory as agent-scope relaxed atomics: global accesses with sc1 set, which are written through
-// to the device's point of coherence and read from it, never from a stale line of another XCD's
-// L2. Each thread's stores are acknowledged (s_waitcnt 0) before the workgroup's barrier that
-// precedes its counter increment, so a workgroup that sees the count also sees the values. No
-// L2 write-back or invalidation is involved, unlike an agent-scope release/acquire fence.
+// memory as 64-bit words stored and loaded as agent-scope relaxed atomics: global accesses with
+// sc1 set, which are written through to the device's point of coherence and read from it, never
+// from a stale line of another XCD's L2. Either a word carries a 32-bit value and a 32-bit tag,
+// and a reader polls it until the tag is the one it expects, so value and tag arrive together;
+// or each writer waits for its stores to be acknowledged (s_waitcnt 0) before its workgroup
+// counts itself in on a counter, and readers poll the counter. Neither involves a fence, an L2
+// write-back or an invalidation.
 // ------------------------------------------------------------------------------------------
-__device__ __forceinline__ void xwg_store(float2* p, float2 v) {
-    uint64_t bits;
-    __builtin_memcpy(&bits, &v, sizeof bits);
+__device__ __forceinline__ uint64_t xwg_word(float v, unsigned tag) {
+    return (static_cast<uint64_t>(tag) << 32) | static_cast<uint64_t>(__float_as_uint(v));
+}
+__device__ __forceinline__ unsigned xwg_tag(uint64_t w) { return static_cast<unsigned>(w >> 32); }
+__device__ __forceinline__ float xwg_value(uint64_t w) { return __uint_as_float(static_cast<unsigned>(w)); }
+__device__ __forceinline__ void xwg_store(uint64_t* p, uint64_t w) {
 #ifdef ISAR_EMU
-    __atomic_store_n(reinterpret_cast<uint64_t*>(p), bits, __ATOMIC_RELAXED);
+    __atomic_store_n(p, w, __ATOMIC_RELAXED);
 #else
-    __hip_atomic_store(reinterpret_cast<uint64_t*>(p), bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #endif
 }
-__device__ __forceinline__ float2 xwg_load(const float2* p) {
+__device__ __forceinline__ uint64_t xwg_load(const uint64_t* p) {
 #ifdef ISAR_EMU
-    const uint64_t bits = __atomic_load_n(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED);
+    return __atomic_load_n(p, __ATOMIC_RELAXED);
 #else
-    const uint64_t bits = __hip_atomic_load(reinterpret_cast<const uint64_t*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #endif
-    float2 v;
-    __builtin_memcpy(&v, &bits, sizeof v);
-    return v;
 }
 // Waits until this thread's stores are acknowledged.
 __device__ __forceinline__ void xwg_stores_done() {
@@ -828,6 +830,13 @@ __device__ __forceinline__ void xwg_arrive(unsigned* c) {
     __hip_atomic_fetch_add(c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 #endif
 }
+__device__ __forceinline__ void xwg_pause() {
+#ifdef ISAR_EMU
+    std::this_thread::yield();
+#else
+    __builtin_amdgcn_s_sleep(2);
+#endif
+}
 // Polls *c until it reaches target; false when `limit` polls pass first, and at once when limit
 // is negative.
 __device__ __forceinline__ bool xwg_wait(const unsigned* c, unsigned target, int limit) {
@@ -835,13 +844,11 @@ __device__ __forceinline__ bool xwg_wait(const unsigned* c, unsigned target, int
     for (int n = 0;; ++n) {
 #ifdef ISAR_EMU
         if (__atomic_load_n(c, __ATOMIC_SEQ_CST) >= target) return true;
-        if (n >= limit) return false;
-        std::this_thread::yield();
 #else
         if (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target) return true;
-        if (n >= limit) return false;
-        __builtin_amdgcn_s_sleep(2);
 #endif
+        if (n >= limit) return false;
+        xwg_pause();
     }
 }
 

@@ -119,6 +119,8 @@ inline float atan2f(float y, float x) { return std::atan2(y, x); }
 inline float rsqrtf(float x) { return 1.0f / std::sqrt(x); }
 inline float __int_as_float(int i) { float f; std::memcpy(&f, &i, 4); return f; }
 inline int __float_as_int(float f) { int i; std::memcpy(&i, &f, 4); return i; }
+inline unsigned __float_as_uint(float f) { unsigned i; std::memcpy(&i, &f, 4); return i; }
+inline float __uint_as_float(unsigned i) { float f; std::memcpy(&f, &i, 4); return f; }
 
 // ---- runtime subset ----
 typedef int hipError_t;

@@ -117,8 +117,8 @@ size_t carve_workspace(const isar_radar_params& p, int K_, char* base, Job* J, S
     j.czt = static_cast<float2*>(frame(M * W * sizeof(float2)));
     j.colsum = static_cast<float2*>(frame(W * sizeof(float2)));
     j.delta = static_cast<float2*>(frame(W * sizeof(float2)));
-    j.af_part = static_cast<float2*>(frame(2 * (W / 8) * M * sizeof(float2)));
-    j.af_sync = static_cast<unsigned*>(frame(sizeof(unsigned)));
+    j.af_part = static_cast<float2*>(frame(4 * (W / 8) * M * sizeof(float2)));   // split: half; resident: all
+    j.af_sync = static_cast<unsigned*>(frame(2 * sizeof(unsigned)));
     SharedTables t{};
     t.czt_table = static_cast<float2*>(shared(W * F * sizeof(float2)));
     t.czt_pre = static_cast<float2*>(shared(W * M * sizeof(float2)));
@@ -391,10 +391,12 @@ int Context::enqueue(int frames, const void* echo, const double* velocity, const
         probed(ISAR_STAGE_PA_COLFFT, s, [&] { launch_pa_colfft(J, s); });
         probed(ISAR_STAGE_PA_SELECT, s, [&] { launch_pa_select(J, s); });
     }
-    const bool af_split = af_mode == ISAR_AF_SPLIT || af_mode == ISAR_AF_SPLIT_PHASE ||
+    // the resident form's exchange tags hold the iteration in 7 bits; above that it runs split
+    const bool af_resident = af_mode == ISAR_AF_RESIDENT && J.iterations <= 128;
+    const bool af_split = af_mode == ISAR_AF_SPLIT || af_mode == ISAR_AF_SPLIT_PHASE || af_mode == ISAR_AF_RESIDENT ||
                           (af_mode == ISAR_AF_AUTO && frames <= kAfSplitFrames);
     probed(ISAR_STAGE_AUTOFOCUS, s, [&] {
-        if (af_mode == ISAR_AF_RESIDENT) launch_autofocus_resident(J, s);
+        if (af_resident) launch_autofocus_resident(J, s);
         else if (af_split) launch_autofocus_split(J, af_mode != ISAR_AF_SPLIT_PHASE, s);
         else launch_autofocus(J, s);
     });

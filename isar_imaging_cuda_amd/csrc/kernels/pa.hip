@@ -214,7 +214,10 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     const int* sel = J.sel + static_cast<size_t>(f) * W;
     float2* su = J.su + static_cast<size_t>(f) * W * M;
     const float2* X = J.x + static_cast<size_t>(f) * M * W;
-    if (tid == 0) J.af_sync[f] = 0;   // the resident form's hand-off counter, for the launch after this one
+    if (tid == 0) {   // the resident form's exchange epoch (the frame's run count) and hand-off counter
+        J.af_sync[2 * f] += 1;
+        J.af_sync[2 * f + 1] = 0;
+    }
 
     // gather the selected bins, column-major (selectRangeBins, :335-346), scaled by 2^-e so that
     // the largest selected |S| lies in [1, 2): the update takes the unit phasor of sums cubic in
@@ -433,22 +436,82 @@ __global__ __launch_bounds__(AfCfg<M>::SUB * AfCfg<M>::TPF) void k_af_cols(Job J
     af_subchunk<M>(s, phi, col, valid, min(SUB, U - c0), bufs, tid, tw, [&](int m, float2 t) { part[m] = t; });
 }
 
+// The exchange words of frame f in the resident form: per iteration parity, sub-chunk q < W/SUB
+// and pulse m, the real and the imaginary part of the sub-chunk's per-pulse sum as tagged words
+// (xwg_word). Frame f owns 4 (W/8) M words of af_part, the buffer being sized for it.
+template <int M>
+__device__ __forceinline__ uint64_t* af_words_of(const Job& J, int f, int it) {
+    const size_t nq = static_cast<size_t>(J.W / AfCfg<M>::SUB);
+    uint64_t* frame = reinterpret_cast<uint64_t*>(J.af_part) + static_cast<size_t>(f) * 4 * (J.W / 8) * M;
+    return frame + static_cast<size_t>(it & 1) * nq * M * 2;
+}
+
+// af_phase_from over tagged words (af_words_of): pulse m's sub-chunk sums are loaded eight at a
+// time, each word polled until it carries `tag`, and added in sub-chunk order, so the phase is
+// af_phase_from's bit for bit. A thread that has polled more than `limit` times gives up for the
+// rest of the launch (waiting = false) and sets status bit 2.
+template <int M>
+__device__ __forceinline__ float2 af_phase_tagged(const uint64_t* words, int nsub, int m, unsigned tag, int limit,
+                                                  bool& waiting, int32_t& status) {
+    float2 acc = make_float2(0.0f, 0.0f);
+    int polls = 0;
+    for (int q0 = 0; q0 < nsub; q0 += 8) {
+        uint64_t re[8], im[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (q0 + i < nsub) {
+                const uint64_t* w = words + (static_cast<size_t>(q0 + i) * M + m) * 2;
+                re[i] = xwg_load(w);
+                im[i] = xwg_load(w + 1);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (q0 + i >= nsub) break;
+            const uint64_t* w = words + (static_cast<size_t>(q0 + i) * M + m) * 2;
+            while (waiting && (xwg_tag(re[i]) != tag || xwg_tag(im[i]) != tag)) {
+                if (polls++ >= limit) {
+                    waiting = false;
+                    atomicOr(&status, 2);
+                    break;
+                }
+                xwg_pause();
+                re[i] = xwg_load(w);
+                im[i] = xwg_load(w + 1);
+            }
+            acc = cadd(acc, make_float2(xwg_value(re[i]), xwg_value(im[i])));
+        }
+    }
+    const float a = cabsf_(acc);
+    return a > 0.0f ? cscale(acc, 1.0f / a) : make_float2(0.0f, 0.0f);
+}
+
 // The fixed-point iterations of k_af_cols (FOLD) in one launch: workgroup (j, f) keeps sub-chunk
 // j of frame f's selected bins in registers for every iteration and hands its per-pulse sums to
-// the frame's other workgroups through af_part and the frame's counter af_sync (xwg_*): after
-// iteration `it` the counter reaches (it + 1) * nsub, and every workgroup then forms the next
-// phase from all nsub sums, in sub-chunk order. The transforms, sums and phases are those of
-// k_af_cols bit for bit. All of a frame's workgroups must be resident together; the dispatcher
-// places a grid's workgroups in order, so every frame but the last one placed is complete, and
-// that one completes as earlier frames finish. A wait that exceeds its bound (J.af_wait_polls
-// polls, about a second by default) sets status bit 2 of the frame and stops that workgroup
-// waiting; the frame's phase is then not valid. Workgroup 0 of each frame writes the final phase
-// to J.phi.
+// the frame's other workgroups; every workgroup then forms the next phase from all nsub sums, in
+// sub-chunk order. The transforms, sums and phases are those of k_af_cols bit for bit.
+// Up to 512 pulses the sums travel as tagged words: each word is polled until it carries
+// iteration it's tag (epoch << 7) | it, where the epoch is the frame's run count that k_autofocus
+// advances before this launch, so words of an earlier run never pass (at most 128 iterations;
+// the host runs the split form above that). At 1024 pulses, where a frame has more sub-chunks and
+// tagged words would double the bytes every workgroup reads per iteration, the sums travel as
+// plain 64-bit words: each thread waits for its stores to be acknowledged, the workgroup counts
+// itself in on the frame's counter (cleared by k_autofocus), and thread 0 polls the counter until
+// all nsub workgroups have counted in for the iteration.
+// Either way a workgroup writes iteration it + 2 into the words of iteration it only after it has
+// read every workgroup's iteration it + 1, which each wrote after reading iteration it.
+// All of a frame's workgroups must be resident together; the dispatcher places a grid's
+// workgroups in order, so every frame but the last one placed is complete, and that one
+// completes as earlier frames finish. A wait that exceeds J.af_wait_polls polls (about a second by
+// default; a negative bound gives up at once) sets status bit 2 of the frame and stops that
+// thread waiting; the frame's phase is then not valid. Workgroup 0 of each frame writes the final
+// phase to J.phi.
 template <int M>
 __global__ __launch_bounds__(AfCfg<M>::SUB * AfCfg<M>::TPF) void k_af_resident(Job J) {
     using A = AfCfg<M>;
     using C = ColCfg<M>;
     constexpr int E = A::E, TPF = A::TPF, SUB = A::SUB, NTS = SUB * TPF;
+    constexpr bool kTagged = M <= 512;
     HIP_DYNAMIC_SHARED(float2, bufs);   // SUB * lds_padded_size(M), then the phase (M)
     float2* phi = bufs + SUB * lds_padded_size(M);
     const int j = blockIdx.x, f = blockIdx.y, tid = threadIdx.x;
@@ -466,23 +529,45 @@ __global__ __launch_bounds__(AfCfg<M>::SUB * AfCfg<M>::TPF) void k_af_resident(J
     for (int e = 0; e < E; ++e) s[e] = valid ? col[C::in(lane, e)] : make_float2(0.0f, 0.0f);
     for (int m = tid; m < M; m += NTS) phi[m] = J.phi[static_cast<size_t>(f) * M + m];
     __syncthreads();
-    bool waiting = true;   // thread 0's: false once a wait has hit its bound
-    unsigned* const count = J.af_sync + f;
+    const unsigned epoch = J.af_sync[2 * f];
+    unsigned* const count = J.af_sync + 2 * f + 1;
+    bool waiting = J.af_wait_polls >= 0;   // false once this thread's polls have hit their bound
+    if (!waiting && tid == 0) atomicOr(&J.dec[f].status, 2);   // a negative bound: every wait gives up
     for (int it = 0; it < J.iterations; ++it) {
-        float2* part = af_part_of<M>(J, f, it);
-        af_subchunk<M>(s, phi, col, valid, min(SUB, U - c0), bufs, tid, tw,
-                       [&](int m, float2 t) { xwg_store(part + static_cast<size_t>(j) * M + m, t); });
-        xwg_stores_done();
-        __syncthreads();
-        if (tid == 0) {
-            xwg_arrive(count);
-            if (waiting && !xwg_wait(count, static_cast<unsigned>(it + 1) * static_cast<unsigned>(nsub), J.af_wait_polls)) {
-                waiting = false;
-                atomicOr(&J.dec[f].status, 2);
+        uint64_t* words = af_words_of<M>(J, f, it);
+        if constexpr (kTagged) {
+            const unsigned tag = (epoch << 7) | static_cast<unsigned>(it);
+            af_subchunk<M>(s, phi, col, valid, min(SUB, U - c0), bufs, tid, tw, [&](int m, float2 t) {
+                uint64_t* w = words + (static_cast<size_t>(j) * M + m) * 2;
+                xwg_store(w, xwg_word(t.x, tag));
+                xwg_store(w + 1, xwg_word(t.y, tag));
+            });
+            for (int m = tid; m < M; m += NTS)
+                phi[m] = af_phase_tagged<M>(words, nsub, m, tag, J.af_wait_polls, waiting, J.dec[f].status);
+        } else {
+            af_subchunk<M>(s, phi, col, valid, min(SUB, U - c0), bufs, tid, tw, [&](int m, float2 t) {
+                uint64_t v;
+                __builtin_memcpy(&v, &t, sizeof v);
+                xwg_store(words + static_cast<size_t>(j) * M + m, v);
+            });
+            xwg_stores_done();
+            __syncthreads();
+            if (tid == 0) {
+                xwg_arrive(count);
+                if (waiting && !xwg_wait(count, static_cast<unsigned>(it + 1) * static_cast<unsigned>(nsub), J.af_wait_polls)) {
+                    waiting = false;
+                    atomicOr(&J.dec[f].status, 2);
+                }
             }
+            __syncthreads();
+            for (int m = tid; m < M; m += NTS)
+                phi[m] = af_phase_from<M>(reinterpret_cast<const float2*>(words), nsub, m, [](const float2* p) {
+                    const uint64_t v = xwg_load(reinterpret_cast<const uint64_t*>(p));
+                    float2 t;
+                    __builtin_memcpy(&t, &v, sizeof t);
+                    return t;
+                });
         }
-        __syncthreads();
-        for (int m = tid; m < M; m += NTS) phi[m] = af_phase_from<M>(part, nsub, m, [](const float2* p) { return xwg_load(p); });
         __syncthreads();
     }
     if (j == 0)

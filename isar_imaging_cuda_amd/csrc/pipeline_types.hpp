@@ -86,8 +86,8 @@ struct Job {
     float2* stm;                  // K x M x W (MTRC range-frequency rows)
     float2* czt;                  // K x M x W (chirp-z output)
     float2* colsum;               // K x W
-    float2* af_part;              // 2 x K x (W/8) x M: per-pulse sums per sub-chunk of selected bins, by iteration parity (split autofocus)
-    unsigned* af_sync;            // K: hand-off counters of the resident autofocus (cleared by its gather launch)
+    float2* af_part;              // K x 4 (W/8) M: per-pulse sums per sub-chunk of selected bins, by iteration parity (split autofocus: the first half; resident: all, as tagged words)
+    unsigned* af_sync;            // 2K: per frame the run count (the resident autofocus's exchange epoch, advanced by k_autofocus) and its hand-off counter (cleared there)
     const float2* czt_table;      // W x F (FFT of the Bluestein chirp per range bin)
     const float2* czt_pre;        // W x M: exp(j pi a_n m (M - m) / M), the input chirp
     const float2* czt_post;       // W x M: exp(-j pi a_n k^2 / M) / F, the output chirp
