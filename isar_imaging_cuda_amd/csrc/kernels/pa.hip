@@ -227,21 +227,48 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     for (int u = tid; u < U; u += NT) cm = fmaxf(cm, J.colmax[static_cast<size_t>(f) * W + sel[u]]);
     cm = block_max<NT>(cm, reinterpret_cast<float*>(red));
     const int e = cm > 0.0f && cm <= 3.40282347e38f ? ilogbf(cm) : 0;
-    for (int idx = tid; idx < U * M; idx += NT) {
-        const int m = idx / U, u = idx % U;
-        const float2 v = X[static_cast<size_t>(m) * W + sel[u]];
-        su[static_cast<size_t>(u) * M + m] = make_float2(ldexpf(v.x, -e), ldexpf(v.y, -e));
+    {   // the selected bins' indices in LDS (the transform buffers are free here), then the
+        // gather with eight loads in flight per thread
+        int* lsel = reinterpret_cast<int*>(bufs);
+        for (int u = tid; u < U; u += NT) lsel[u] = sel[u];
+        __syncthreads();
+        for (int idx0 = tid; idx0 < U * M; idx0 += 8 * NT) {
+            float2 v[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int idx = idx0 + i * NT;
+                if (idx < U * M) v[i] = X[static_cast<size_t>(idx / U) * W + lsel[idx % U]];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int idx = idx0 + i * NT;
+                if (idx < U * M)
+                    su[static_cast<size_t>(idx % U) * M + idx / U] = make_float2(ldexpf(v[i].x, -e), ldexpf(v[i].y, -e));
+            }
+        }
     }
     __syncthreads();
     if (J.phi0_given) {
         for (int m = tid; m < M; m += NT) phi[m] = J.phi0_in[static_cast<size_t>(f) * M + m];
         __syncthreads();
     } else {
-        // Doppler tracking of the selected bins without compensation (:282)
+        // Doppler tracking of the selected bins without compensation (:282), the products added
+        // in bin order with eight bins' loads in flight
         for (int m = tid; m < M - 1; m += NT) {
             float2 acc = make_float2(0.0f, 0.0f);
-            for (int u = 0; u < U; ++u)
-                acc = cadd(acc, cconjmul(su[static_cast<size_t>(u) * M + m], su[static_cast<size_t>(u) * M + m + 1]));
+            for (int u0 = 0; u0 < U; u0 += 8) {
+                float2 a[8], b[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    if (u0 + i < U) {
+                        a[i] = su[static_cast<size_t>(u0 + i) * M + m];
+                        b[i] = su[static_cast<size_t>(u0 + i) * M + m + 1];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i)
+                    if (u0 + i < U) acc = cadd(acc, cconjmul(a[i], b[i]));
+            }
             rdop[m] = acc;
         }
         __syncthreads();
