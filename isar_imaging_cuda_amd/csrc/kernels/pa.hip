@@ -371,15 +371,15 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
 template <int M, typename Load>
 __device__ __forceinline__ float2 af_phase_from(const float2* part, int nsub, int m, Load load) {
     float2 acc = make_float2(0.0f, 0.0f);
-    int q = 0;
-    for (; q + 8 <= nsub; q += 8) {   // eight loads in flight, added in order
+    for (int q = 0; q < nsub; q += 8) {   // up to eight loads in flight, added in order
         float2 v[8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = load(part + static_cast<size_t>(q + i) * M + m);
+        for (int i = 0; i < 8; ++i)
+            if (q + i < nsub) v[i] = load(part + static_cast<size_t>(q + i) * M + m);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) acc = cadd(acc, v[i]);
+        for (int i = 0; i < 8; ++i)
+            if (q + i < nsub) acc = cadd(acc, v[i]);
     }
-    for (; q < nsub; ++q) acc = cadd(acc, load(part + static_cast<size_t>(q) * M + m));
     const float a = cabsf_(acc);
     return a > 0.0f ? cscale(acc, 1.0f / a) : make_float2(0.0f, 0.0f);
 }
