@@ -328,6 +328,9 @@ struct Dft<16, DIR> {
 // the last-pass store are both unit-stride across threads (coalesced against global memory).
 // Between passes the data goes through an LDS buffer with one padding slot every 16 elements.
 // All threads of the workgroup must call the transform together (it uses __syncthreads).
+// With WAVE = true the passes are ordered by fft_sync<true> instead: for transforms of at most
+// 64 threads, aligned within a wave, that touch only their own LDS buffer (N/E <= 64 and
+// 64 % (N/E) == 0); the caller orders other waves' accesses to that buffer with barriers of its own.
 // ------------------------------------------------------------------------------------------
 // A copy of v that the compiler cannot see through. Index and address arithmetic derived from it
 // is recomputed where it is used instead of being kept live in registers across transforms.
@@ -375,6 +378,29 @@ __device__ __forceinline__ float2 opaque(float2 v) {
 // than a 128-byte line then share its fetch in one L2 instead of each XCD fetching it.
 __device__ __forceinline__ int xcd_tile(int b, int tiles) {
     return (tiles & 7) ? b : (b & 7) * (tiles >> 3) + (b >> 3);
+}
+
+// Orders the LDS accesses of an FFT's passes. WAVE = false: a workgroup barrier. WAVE = true: the
+// transform lives in one wave, whose LDS instructions the hardware performs in issue order, so
+// only the compiler has to be kept from moving accesses across (a wavefront-scope fence needs no
+// wait and no barrier instruction).
+template <bool WAVE>
+__device__ __forceinline__ void fft_sync() {
+#ifdef ISAR_EMU
+    __syncthreads();   // every thread of the block calls the transform together
+#else
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+#endif
+}
+template <int N, int E, bool WAVE>
+__device__ __forceinline__ constexpr void fft_check_wave() {
+    static_assert(!WAVE || (N / E <= 64 && 64 % (N / E) == 0), "a wave-ordered transform must lie in one wave");
 }
 
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
@@ -539,20 +565,21 @@ template <> struct FftPlan<192, 12>   { static constexpr int NP = 4; static cons
 template <> struct FftPlan<384, 12>   { static constexpr int NP = 5; static constexpr int R[5] = {4, 4, 4, 2, 3}; };
 template <> struct FftPlan<768, 12>   { static constexpr int NP = 5; static constexpr int R[5] = {4, 4, 4, 4, 3}; };
 
-template <int N, int E, int DIR, int P, int Ns>
+template <int N, int E, int DIR, int P, int Ns, bool WAVE = false>
 struct FftPasses {
     using Plan = FftPlan<N, E>;
     // runs passes P..NP-1; x holds the layout of pass P on entry
     __device__ __forceinline__ static void run(float2 (&x)[E], float2* lds, int tid) {
+        fft_check_wave<N, E, WAVE>();
         constexpr int R = Plan::R[P];
         stockham_compute<N, E, R, DIR, Ns>(x, tid);
         if constexpr (P + 1 < Plan::NP) {
             constexpr int RN = Plan::R[P + 1];
             stockham_store<N, E, R, Ns>(x, lds, tid);
-            __syncthreads();
+            fft_sync<WAVE>();
             stockham_load<N, E, RN>(x, lds, tid);
-            __syncthreads();
-            FftPasses<N, E, DIR, P + 1, Ns * R>::run(x, lds, tid);
+            fft_sync<WAVE>();
+            FftPasses<N, E, DIR, P + 1, Ns * R, WAVE>::run(x, lds, tid);
         }
     }
 };
@@ -561,11 +588,12 @@ struct FftPasses {
 // The radix-16 pass at stride 16 needs only the 256 twiddles exp(j 2 pi m / 256): a kernel that
 // runs many transforms may keep them in LDS (fft_tw16_table_fill) and read them there
 // (TAB = true) instead of forming the 15 powers from the cached bases at every transform.
-template <int N, int E, int DIR, int P, int Ns, bool TAB = false>
+template <int N, int E, int DIR, int P, int Ns, bool TAB = false, bool WAVE = false>
 struct FftPassesCached {
     using Plan = FftPlan<N, E>;
     __device__ __forceinline__ static void run(float2 (&x)[E], float2* lds, int tid, const FftTwiddles<N, E>& tw,
                                                const float2* tab = nullptr) {
+        fft_check_wave<N, E, WAVE>();
         constexpr int R = Plan::R[P];
         if constexpr (E == R) {   // one radix group per thread: twiddles from the cache
             if constexpr (TAB && Ns == 16 && R == 16) {
@@ -585,10 +613,10 @@ struct FftPassesCached {
         if constexpr (P + 1 < Plan::NP) {
             constexpr int RN = Plan::R[P + 1];
             stockham_store<N, E, R, Ns>(x, lds, tid);
-            __syncthreads();
+            fft_sync<WAVE>();
             stockham_load<N, E, RN>(x, lds, tid);
-            __syncthreads();
-            FftPassesCached<N, E, DIR, P + 1, Ns * R, TAB>::run(x, lds, tid, tw, tab);
+            fft_sync<WAVE>();
+            FftPassesCached<N, E, DIR, P + 1, Ns * R, TAB, WAVE>::run(x, lds, tid, tw, tab);
         }
     }
 };
@@ -608,10 +636,11 @@ struct FftTwiddlePowers {
     float2 w[5][E];   // rows of the other passes are never formed or read
 };
 
-template <int N, int E, int DIR, int P, int Ns>
+template <int N, int E, int DIR, int P, int Ns, bool WAVE = false>
 struct FftPassesPow {
     using Plan = FftPlan<N, E>;
     __device__ __forceinline__ static void run(float2 (&x)[E], float2* lds, int tid, const FftTwiddlePowers<N, E>& pw) {
+        fft_check_wave<N, E, WAVE>();
         constexpr int R = Plan::R[P];
         if constexpr (E == R) {
             if constexpr (Ns > 1) {
@@ -628,10 +657,10 @@ struct FftPassesPow {
         if constexpr (P + 1 < Plan::NP) {
             constexpr int RN = Plan::R[P + 1];
             stockham_store<N, E, R, Ns>(x, lds, tid);
-            __syncthreads();
+            fft_sync<WAVE>();
             stockham_load<N, E, RN>(x, lds, tid);
-            __syncthreads();
-            FftPassesPow<N, E, DIR, P + 1, Ns * R>::run(x, lds, tid, pw);
+            fft_sync<WAVE>();
+            FftPassesPow<N, E, DIR, P + 1, Ns * R, WAVE>::run(x, lds, tid, pw);
         }
     }
 };
@@ -675,14 +704,14 @@ __device__ __forceinline__ FftTwiddles<N, E> fft_twiddles(int tid) {
 
 // In-register/LDS transform. On entry x[e] = in[fft_in_pos(tid, e)], on exit
 // x[e] = out[fft_out_pos(tid, e)]. lds must hold lds_padded_size(N) float2 for this transform.
-template <int N, int E, int DIR>
+template <int N, int E, int DIR, bool WAVE = false>
 __device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid) {
-    FftPasses<N, E, DIR, 0, 1>::run(x, lds, tid);
+    FftPasses<N, E, DIR, 0, 1, WAVE>::run(x, lds, tid);
 }
 // The same transform with twiddle bases from fft_twiddles<N, E>(tid).
-template <int N, int E, int DIR>
+template <int N, int E, int DIR, bool WAVE = false>
 __device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid, const FftTwiddles<N, E>& tw) {
-    FftPassesCached<N, E, DIR, 0, 1>::run(x, lds, tid, tw);
+    FftPassesCached<N, E, DIR, 0, 1, false, WAVE>::run(x, lds, tid, tw);
 }
 // ... and the stride-16 radix-16 pass's twiddles from an LDS table (fft_tw16_table_fill).
 template <int N, int E, int DIR>
@@ -697,9 +726,9 @@ __device__ __forceinline__ FftTwiddlePowers<N, E> fft_twiddle_powers(const FftTw
     fft_twiddle_powers_fill<N, E, 0, 1>(pw, tw);
     return pw;
 }
-template <int N, int E, int DIR>
+template <int N, int E, int DIR, bool WAVE = false>
 __device__ __forceinline__ void fft(float2 (&x)[E], float2* lds, int tid, const FftTwiddlePowers<N, E>& pw) {
-    FftPassesPow<N, E, DIR, 0, 1>::run(x, lds, tid, pw);
+    FftPassesPow<N, E, DIR, 0, 1, WAVE>::run(x, lds, tid, pw);
 }
 
 template <int N, int E>
@@ -771,18 +800,20 @@ struct ColCfg {
     __device__ static int out(int lane, int e) { return fft_out_pos<M, E, RL>(lane, e); }
 };
 
-// Move a transform result from the last-pass layout to the first-pass layout through LDS.
-template <int M, int E>
+// Move a transform result from the last-pass layout to the first-pass layout through LDS
+// (WAVE: ordered as the transforms with fft_sync<WAVE>).
+template <int M, int E, bool WAVE = false>
 __device__ __forceinline__ void relayout_out_to_in(float2 (&x)[E], float2* buf, int lane) {
+    fft_check_wave<M, E, WAVE>();
     constexpr int R0 = FftPlan<M, E>::R[0];
     constexpr int RL = FftPlan<M, E>::R[FftPlan<M, E>::NP - 1];
     if constexpr (R0 != RL) {
 #pragma unroll
         for (int e = 0; e < E; ++e) buf[lds_pad(fft_out_pos<M, E, RL>(lane, e))] = x[e];
-        __syncthreads();
+        fft_sync<WAVE>();
 #pragma unroll
         for (int e = 0; e < E; ++e) x[e] = buf[lds_pad(fft_in_pos<M, E, R0>(lane, e))];
-        __syncthreads();
+        fft_sync<WAVE>();
     }
 }
 

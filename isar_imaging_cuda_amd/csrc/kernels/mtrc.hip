@@ -96,9 +96,9 @@ __global__ __launch_bounds__(kRowsPerBlock * W / kRowE) void k_mtrc_rows(Job J) 
     for (int e = 0; e < kRowE; ++e) x[e] = cmul(xr[(row_in<W>(lane, e) + W / 2) & (W - 1)], ph);
     float2* buf = sm + g * lds_padded_size(W);
     if (J.data_type == 1) {
-        fft<W, kRowE, -1>(x, buf, lane);
+        fft<W, kRowE, -1, true>(x, buf, lane);
     } else {
-        fft<W, kRowE, +1>(x, buf, lane);
+        fft<W, kRowE, +1, true>(x, buf, lane);
         const float inv = 1.0f / W;
 #pragma unroll
         for (int e = 0; e < kRowE; ++e) x[e] = cscale(x[e], inv);
@@ -137,14 +137,17 @@ __global__ __launch_bounds__(NTW) void k_mtrc_czt(Job J) {
         x[e] = pos < M ? cmul(sm[pos * (CT + 1) + g], pre[pos]) : make_float2(0.0f, 0.0f);
     }
     __syncthreads();
+    // a transform within one wave orders its passes by the wave alone (the transpose back below
+    // waits for every wave with a barrier of its own)
+    constexpr bool kWave = TPF <= 64;
     float2* buf = sm + g * lds_padded_size(F);
     const FftTwiddles<F, E> tw = fft_twiddles<F, E>(lane);
-    fft<F, E, -1>(x, buf, lane, tw);
+    fft<F, E, -1, kWave>(x, buf, lane, tw);
     const float2* tab = J.czt_table + static_cast<size_t>(n) * F;
 #pragma unroll
     for (int e = 0; e < E; ++e) x[e] = cmul(x[e], tab[C::out(lane, e)]);
-    relayout_out_to_in<F, E>(x, buf, lane);
-    fft<F, E, +1>(x, buf, lane, tw);
+    relayout_out_to_in<F, E, kWave>(x, buf, lane);
+    fft<F, E, +1, kWave>(x, buf, lane, tw);
     float2 csum = make_float2(0.0f, 0.0f);
     float2 y[E];
 #pragma unroll
@@ -159,6 +162,7 @@ __global__ __launch_bounds__(NTW) void k_mtrc_czt(Job J) {
     if constexpr (TPF <= 64) {
         csum = group_sum2<TPF>(csum);
         if (lane == 0) J.colsum[static_cast<size_t>(f) * W + n] = csum;
+        __syncthreads();   // every wave's last transform loads are done before the transpose
     } else {
         // one transform spans TPF / 64 waves: wave sums first, then the transform's waves in LDS.
         // The partials sit past the transpose area (M (CT + 1) entries); czt_lds counts them.
@@ -270,7 +274,7 @@ __global__ __launch_bounds__(kRowsPerBlock * W / kRowE) void k_mtrc_image(Job J)
     float2 x[kRowE];
 #pragma unroll
     for (int e = 0; e < kRowE; ++e) x[e] = cr[row_in<W>(lane, e)];
-    fft<W, kRowE, -1>(x, buf, lane);
+    fft<W, kRowE, -1, true>(x, buf, lane);
     if (valid) {
         float* dst = image_of(J, f) + static_cast<size_t>(k) * W;
 #pragma unroll

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(256) void k_pa_colfft(Job J) {
     }
     __syncthreads();
     float2* buf = work + g * lds_padded_size(M);
-    fft<M, E, -1>(x, buf, lane);
+    fft<M, E, -1, true>(x, buf, lane);
     float p2[E];
     float esum = 0.0f;
 #pragma unroll
@@ -298,11 +298,11 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
             float2 x[E];
 #pragma unroll
             for (int e = 0; e < E; ++e) x[e] = cmul(phi[C::in(lane, e)], s[e]);
-            fft<M, E, -1>(x, buf, lane, tw);
+            fft<M, E, -1, true>(x, buf, lane, tw);
 #pragma unroll
             for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabs_hw(x[e]));
-            relayout_out_to_in<M, E>(x, buf, lane);
-            fft<M, E, +1>(x, buf, lane, tw);
+            relayout_out_to_in<M, E, true>(x, buf, lane);
+            fft<M, E, +1, true>(x, buf, lane, tw);
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 const int pos = C::out(lane, e);
@@ -413,11 +413,11 @@ __device__ __forceinline__ void af_subchunk(const float2 (&s)[AfCfg<M>::E], cons
     float2 x[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) x[e] = cmul(phi[C::in(lane, e)], s[e]);
-    fft<M, E, -1>(x, buf, lane, tw);
+    fft<M, E, -1, true>(x, buf, lane, tw);
 #pragma unroll
     for (int e = 0; e < E; ++e) x[e] = cscale(x[e], cabs_hw(x[e]));
-    relayout_out_to_in<M, E>(x, buf, lane);
-    fft<M, E, +1>(x, buf, lane, tw);
+    relayout_out_to_in<M, E, true>(x, buf, lane);
+    fft<M, E, +1, true>(x, buf, lane, tw);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int pos = C::out(lane, e);
