@@ -340,6 +340,15 @@ __device__ __forceinline__ int opaque(int v) {
 #endif
     return v;
 }
+// Element i of p for 0 <= i < 2^28 / sizeof(T) (p typically wave-uniform): the byte offset is
+// formed in 32 bits, so the access can take the base in scalar registers and a 32-bit per-lane
+// offset instead of a 64-bit address per lane.
+template <typename T>
+__device__ __forceinline__ T& elem_u32(T* p, unsigned i) {
+    using B = std::conditional_t<std::is_const_v<T>, const char, char>;
+    return *reinterpret_cast<T*>(reinterpret_cast<B*>(p) + i * static_cast<unsigned>(sizeof(T)));
+}
+
 // Value of v held by lane `lane` (a compile-time constant in practice) of the calling wave.
 __device__ __forceinline__ float wave_read_lane(float v, int lane) {
 #ifdef ISAR_EMU

@@ -54,7 +54,7 @@ __device__ __forceinline__ void load_row_raw(float2 (&x)[kE], const Job& J, int 
     const bool hpc = J.if_hpc && J.data_type == 2;
     const bool hrrp_window = !ONE_WINDOW && J.if_hrrp && J.data_type == 2;
     auto window = [&](int n) {
-        float w = J.win_range[n];
+        float w = elem_u32(J.win_range, n);
         if (hrrp_window) w *= w;
         return (n & 1) ? -w : w;
     };
@@ -62,7 +62,7 @@ __device__ __forceinline__ void load_row_raw(float2 (&x)[kE], const Job& J, int 
 #pragma unroll
         for (int e = 0; e < kE; ++e) {
             const int n = in_pos<N>(tid, e);
-            x[e] = cscale(row[n], window(n));
+            x[e] = cscale(elem_u32(row, n), window(n));
         }
         return;
     }
@@ -83,7 +83,7 @@ __device__ __forceinline__ void load_row_raw(float2 (&x)[kE], const Job& J, int 
             const int e = 4 * q + r;
             const int n = in_pos<N>(tid, e);
             const float2 ce = make_float2(wave_read_lane(c.x, e), wave_read_lane(c.y, e));
-            x[e] = cscale(cmul(row[n], cmul(h, ce)), window(n));
+            x[e] = cscale(cmul(elem_u32(row, n), cmul(h, ce)), window(n));
             if (r < 3) h = cmul(h, d1);
         }
         if (q + 1 < kE / 4) base = cmul(base, d4);
@@ -114,7 +114,7 @@ __device__ __forceinline__ void load_row_shifted(float2 (&x)[kE], const Job& J, 
     const int tid = opaque(tid_);
     const float2* row = J.pre + (static_cast<size_t>(f) * J.M + m) * N;
 #pragma unroll
-    for (int e = 0; e < kE; ++e) x[e] = row[in_pos<N>(tid, e)];
+    for (int e = 0; e < kE; ++e) x[e] = elem_u32(row, static_cast<unsigned>(in_pos<N>(tid, e)));
     apply_ramp<N>(x, lag, tid);
 }
 
@@ -126,7 +126,7 @@ __device__ __forceinline__ void group_energy(float (&a)[kE], const Job& J, int f
     const int M = J.M;
     const float* base = J.egrp + (static_cast<size_t>(f) * M + static_cast<size_t>(g) * s) * N;
 #pragma unroll
-    for (int e = 0; e < kE; ++e) a[e] = base[in_pos<N>(tid, e)];
+    for (int e = 0; e < kE; ++e) a[e] = elem_u32(base, in_pos<N>(tid, e));
     if (s > 2) {
         const int s2 = s / 2;
         const int r0 = (2 * g + 1) * s2;
@@ -136,7 +136,7 @@ __device__ __forceinline__ void group_energy(float (&a)[kE], const Job& J, int f
             for (int h = r0; h < r1; h += R) {
                 const float* pp = J.epart + (static_cast<size_t>(f) * M + h) * N;
 #pragma unroll
-                for (int e = 0; e < kE; ++e) a[e] += pp[in_pos<N>(tid, e)];
+                for (int e = 0; e < kE; ++e) a[e] += elem_u32(pp, in_pos<N>(tid, e));
             }
         }
     }
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
     {
         const int t = opaque(tid);
 #pragma unroll
-        for (int e = 0; e < kE; ++e) s0a[in_pos<N>(t, e)] = x[e];
+        for (int e = 0; e < kE; ++e) elem_u32(s0a, in_pos<N>(t, e)) = x[e];
     }
     fft<N, kE, +1>(x, lds, tid, tw, tab);
     {
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
 #pragma unroll
         for (int e = 0; e < kE; ++e) {
             ea[e] = cabs2(x[e]) * inv2;
-            dst[out_pos<N>(t, e)] = ea[e];
+            elem_u32(dst, out_pos<N>(t, e)) = ea[e];
         }
     }
     if (final_rows) {   // row 0 is the reference of the merge tree and never moves
@@ -351,9 +351,9 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
         for (int e = 0; e < kE; ++e) {
             const float2 v = cscale(x[e], invN);
             const int pos = out_pos<N>(t, e);
-            s0a[pos] = v;
+            elem_u32(s0a, pos) = v;
             const float av = cabs_hw(v);
-            ap[pos] = av;
+            elem_u32(ap, pos) = av;
             mx = fmaxf(mx, av);
         }
     }
@@ -362,7 +362,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
     {
         const int t = opaque(tid);
 #pragma unroll
-        for (int e = 0; e < kE; ++e) s0b[in_pos<N>(t, e)] = x[e];
+        for (int e = 0; e < kE; ++e) elem_u32(s0b, in_pos<N>(t, e)) = x[e];
     }
     fft<N, kE, +1>(x, lds, tid, tw, tab);
     // Above 4096 cells the two energy rows do not stay in registers across the correlation's
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
     if constexpr (kStash) {
         const int t = opaque(tid);
 #pragma unroll
-        for (int e = 0; e < kE; ++e) eb_row[out_pos<N>(t, e)] = cabs2(x[e]) * inv2;
+        for (int e = 0; e < kE; ++e) elem_u32(eb_row, out_pos<N>(t, e)) = cabs2(x[e]) * inv2;
     }
     // the correlation takes the energies in the first-pass layout
 #pragma unroll
@@ -390,8 +390,8 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
         const int t = opaque(tid);
 #pragma unroll
         for (int e = 0; e < kE; ++e) {
-            ta[e] = dst[out_pos<N>(t, e)];
-            tb[e] = eb_row[out_pos<N>(t, e)];
+            ta[e] = elem_u32(dst, out_pos<N>(t, e));
+            tb[e] = elem_u32(eb_row, out_pos<N>(t, e));
         }
         lag = parabola_lag<N, true>(ta, tb, bi, lds, tid, J.lags + idx, J.imax + idx);
     } else {
@@ -403,7 +403,7 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
     {
         const int t = opaque(tid);
 #pragma unroll
-        for (int e = 0; e < kE; ++e) dst[out_pos<N>(t, e)] += cabs2(x[e]) * inv2;
+        for (int e = 0; e < kE; ++e) elem_u32(dst, out_pos<N>(t, e)) += cabs2(x[e]) * inv2;
     }
     if (tid == 0) {
         J.lagsum[static_cast<size_t>(f) * M + ra] = 0.0;
@@ -415,9 +415,9 @@ __global__ __launch_bounds__(N / kE, 4) void k_ra_pair0(Job J) {
         for (int e = 0; e < kE; ++e) {
             const float2 v = cscale(x[e], invN);
             const int pos = out_pos<N>(t, e);
-            s0b[pos] = v;
+            elem_u32(s0b, pos) = v;
             const float av = cabs_hw(v);
-            ap[pos] += av;
+            elem_u32(ap, pos) += av;
             mx = fmaxf(mx, av);
         }
         mx = block_max<NT>(mx, redv);
@@ -445,7 +445,7 @@ __global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base)
     {   // keep the first group's energy for the next level (same head row, same thread)
         float* dst = J.egrp + (static_cast<size_t>(f) * M + static_cast<size_t>(gA) * s) * N;
 #pragma unroll
-        for (int e = 0; e < kE; ++e) dst[in_pos<N>(tid, e)] = a[e];
+        for (int e = 0; e < kE; ++e) elem_u32(dst, in_pos<N>(tid, e)) = a[e];
     }
     if (unpaired) return;
     group_energy<N>(b, J, f, gA + 1, s, tid);
@@ -457,7 +457,7 @@ __global__ __launch_bounds__(N / kE) void k_ra_corr(Job J, int s, int pair_base)
         const float* src = J.egrp + (static_cast<size_t>(f) * M + static_cast<size_t>(gA) * s) * N;
         const int t = opaque(tid);
 #pragma unroll
-        for (int e = 0; e < kE; ++e) a[e] = src[in_pos<N>(t, e)];
+        for (int e = 0; e < kE; ++e) a[e] = elem_u32(src, in_pos<N>(t, e));
     }
     group_energy<N>(b, J, f, gA + 1, s, opaque(tid));
     (void)parabola_lag<N>(a, b, bi, lds, tid, J.lags + idx, J.imax + idx);
@@ -519,10 +519,10 @@ __global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) voi
             for (int e = 0; e < kE; ++e) {
                 const float2 v = cscale(x[e], invN);
                 const int pos = out_pos<N>(t, e);
-                st[pos] = v;
+                elem_u32(st, pos) = v;
                 const float av = cabs_hw(v);
                 if constexpr (kArpInRegs) arp[e] += av;
-                else ap[pos] = (m == r0 ? 0.0f : ap[pos]) + av;
+                else elem_u32(ap, pos) = (m == r0 ? 0.0f : elem_u32(ap, pos)) + av;
                 mx = fmaxf(mx, av);
             }
         }
@@ -531,7 +531,7 @@ __global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) voi
         if constexpr (kArpInRegs) {
             const int t = opaque(tid);
 #pragma unroll
-            for (int e = 0; e < kE; ++e) ap[out_pos<N>(t, e)] = arp[e];
+            for (int e = 0; e < kE; ++e) elem_u32(ap, out_pos<N>(t, e)) = arp[e];
         }
         float* red = reinterpret_cast<float*>(lds + lds_padded_size(N));
         mx = block_max<N / kE>(mx, red);
@@ -541,7 +541,7 @@ __global__ __launch_bounds__(N / kE, PART == kUpdFinal && N <= 4096 ? 3 : 4) voi
     if (tid < r1 - r0) J.lagsum[static_cast<size_t>(f) * M + r0 + tid] += lag;
     float* dst = J.epart + (static_cast<size_t>(f) * M + r0) * N;
 #pragma unroll
-    for (int e = 0; e < kE; ++e) dst[out_pos<N>(tid, e)] = acc[e];
+    for (int e = 0; e < kE; ++e) elem_u32(dst, out_pos<N>(tid, e)) = acc[e];
 }
 
 // Range Hamming window 0.54 - 0.46 cos(2 pi n / (N - 1)) (genHammingVec, common.cu:260-266),
