@@ -195,11 +195,13 @@ def learned_vs_plain(frames: int = 1000) -> dict:
 
 
 # per pulse count of the sub-aperture plan: chunk rows, autofocus form, chirp-z length and
-# threads, as the plan's autotuner chose them for this workload on an MI355X
+# threads, as the plan's autotuner chose them for this workload on an MI355X, then the autofocus
+# form of the 192- and 384-pulse groups from a sweep of forms and chunk sizes per group, each
+# table timed by graph replay with the groups placed on branches by measured times
 SA_TUNING = {128: dict(ra_chunk=2, autofocus_mode=3, czt_length=256, czt_threads=256),
-             192: dict(ra_chunk=4, autofocus_mode=3, czt_length=512, czt_threads=256),
+             192: dict(ra_chunk=4, autofocus_mode=1, czt_length=512, czt_threads=256),
              256: dict(ra_chunk=4, autofocus_mode=1, czt_length=512, czt_threads=256),
-             384: dict(ra_chunk=4, autofocus_mode=3, czt_length=1024, czt_threads=256),
+             384: dict(ra_chunk=4, autofocus_mode=1, czt_length=1024, czt_threads=256),
              512: dict(ra_chunk=4, autofocus_mode=3, czt_length=1024, czt_threads=256)}
 
 

@@ -262,6 +262,12 @@ ISAR_API int isar_get_trace(isar_context* ctx, int frame, isar_frame_trace* out)
  * +1 inverse); n in {64,128,192,256,384,512,1024,2048,4096,8192,16384}. Synchronous. */
 ISAR_API int isar_fft_host(int n, int dir, const float* h_in, float* h_out, int batch);
 
+/* The same transforms arranged as the autofocus, chirp-z and MTRC kernels run them: several per
+ * 256-thread workgroup, each within one wave and ordered by that wave alone; n in
+ * {64,128,192,256,384,512,1024}. The results equal isar_fft_host's bit for bit. No counterpart in
+ * the reference (a test entry point). Synchronous. */
+ISAR_API int isar_fft_host_grouped(int n, int dir, const float* h_in, float* h_out, int batch);
+
 /* Kernel families that can be probed with HIP events (isar_probe_enable). */
 enum isar_stage {
     ISAR_STAGE_IFDS_PRE = 0, ISAR_STAGE_RA_PAIR0 = 1, ISAR_STAGE_RA_CORR = 2, ISAR_STAGE_RA_UPD = 3,

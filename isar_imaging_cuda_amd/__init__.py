@@ -410,12 +410,17 @@ def run_device_static(params: RadarParams, echo_ptr: int, velocity_ptr: int | No
                                     vp(img_ptr), C.byref(options.c()), vp(stream or 0)), "isar_run_device_static")
 
 
-def fft(x: np.ndarray, inverse: bool = False) -> np.ndarray:
-    """Unnormalised device FFT of each row of a 2-D complex array (tests and diagnostics)."""
+def fft(x: np.ndarray, inverse: bool = False, grouped: bool = False) -> np.ndarray:
+    """Unnormalised device FFT of each row of a 2-D complex array (tests and diagnostics). With
+    grouped=True the rows are transformed several per workgroup, each within one wave
+    (isar_fft_host_grouped, lengths 64 ... 1024)."""
     a = np.ascontiguousarray(x, dtype=np.complex64)
     if a.ndim == 1:
         a = a[None]
     out = np.zeros_like(a)
-    _check(lib().isar_fft_host(a.shape[1], 1 if inverse else -1, a.ctypes.data_as(C.c_void_p),
-                               out.ctypes.data_as(C.c_void_p), a.shape[0]), "isar_fft_host")
+    name = "isar_fft_host_grouped" if grouped else "isar_fft_host"
+    fn = getattr(lib(), name)
+    fn.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_int]
+    _check(fn(a.shape[1], 1 if inverse else -1, a.ctypes.data_as(C.c_void_p), out.ctypes.data_as(C.c_void_p),
+              a.shape[0]), name)
     return out

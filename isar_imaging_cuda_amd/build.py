@@ -36,8 +36,9 @@ HIP_FLAGS = [
 ]
 # Per-source additions. pa.hip: multiply-adds are fused only where one source expression has
 # them, so the one-workgroup and the split autofocus kernels, which inline the same transform
-# code into different surroundings, round identically.
-HIP_FILE_FLAGS = {"pa.hip": ["-ffp-contract=on"]}
+# code into different surroundings, round identically. fft_debug.hip: the same, so that its
+# one-transform-per-workgroup and its wave-grouped test kernels can be compared bit for bit.
+HIP_FILE_FLAGS = {"pa.hip": ["-ffp-contract=on"], "fft_debug.hip": ["-ffp-contract=on"]}
 HOST_FLAGS = ["-O3", "-std=c++20", "-fPIC", "-Wall", "-fopenmp"]
 
 

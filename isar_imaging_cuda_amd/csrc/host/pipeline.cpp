@@ -573,6 +573,37 @@ int Context::trace(int frame, isar_frame_trace* out) {
 // ------------------------------------------------------------------------------------------
 using isar::fail;
 
+namespace {
+// Host-fed device transforms for tests: one per workgroup (debug_fft) or several per workgroup,
+// each within one wave (debug_fft_grouped).
+int fft_host(int n, int dir, const float* h_in, float* h_out, int batch, bool grouped) {
+    if (!h_in || !h_out || batch < 1 || (dir != -1 && dir != 1)) return fail(ISAR_ERR_INVALID, "bad fft arguments");
+    hipError_t ce = isar::configure_kernels();
+    if (ce != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("kernel configuration failed: ") + isar::g_config_error);
+    const size_t bytes = static_cast<size_t>(n) * batch * sizeof(float2);
+    void *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(ISAR_ERR_MEMORY, "fft buffers");
+    }
+    hipError_t e = hipMemcpy(a, h_in, bytes, hipMemcpyHostToDevice);
+    const auto* in = static_cast<const float2*>(a);
+    auto* out = static_cast<float2*>(b);
+    bool ok = e == hipSuccess && (grouped ? isar::debug_fft_grouped(n, dir, in, out, batch, nullptr)
+                                          : isar::debug_fft(n, dir, in, out, batch, nullptr));
+    if (ok) {
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpy(h_out, b, bytes, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+    }
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (!ok) return fail(ISAR_ERR_INVALID, "unsupported fft length " + std::to_string(n));
+    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("fft: ") + hipGetErrorString(e));
+    return ISAR_OK;
+}
+}  // namespace
+
 extern "C" {
 
 ISAR_API void isar_default_options(isar_options* opt) {
@@ -789,28 +820,13 @@ ISAR_API int isar_get_trace(isar_context* ctx, int frame, isar_frame_trace* out)
     return ctx->trace(frame, out);
 }
 
+
 ISAR_API int isar_fft_host(int n, int dir, const float* h_in, float* h_out, int batch) {
-    if (!h_in || !h_out || batch < 1 || (dir != -1 && dir != 1)) return fail(ISAR_ERR_INVALID, "bad fft arguments");
-    hipError_t ce = isar::configure_kernels();
-    if (ce != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("kernel configuration failed: ") + isar::g_config_error);
-    const size_t bytes = static_cast<size_t>(n) * batch * sizeof(float2);
-    void *a = nullptr, *b = nullptr;
-    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
-        (void)hipFree(a);
-        return fail(ISAR_ERR_MEMORY, "fft buffers");
-    }
-    hipError_t e = hipMemcpy(a, h_in, bytes, hipMemcpyHostToDevice);
-    bool ok = e == hipSuccess && isar::debug_fft(n, dir, static_cast<const float2*>(a), static_cast<float2*>(b), batch, nullptr);
-    if (ok) {
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpy(h_out, b, bytes, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipDeviceSynchronize();
-    }
-    (void)hipFree(a);
-    (void)hipFree(b);
-    if (!ok) return fail(ISAR_ERR_INVALID, "unsupported fft length " + std::to_string(n));
-    if (e != hipSuccess) return fail(ISAR_ERR_DEVICE, std::string("fft: ") + hipGetErrorString(e));
-    return ISAR_OK;
+    return fft_host(n, dir, h_in, h_out, batch, false);
+}
+
+ISAR_API int isar_fft_host_grouped(int n, int dir, const float* h_in, float* h_out, int batch) {
+    return fft_host(n, dir, h_in, h_out, batch, true);
 }
 
 ISAR_API int isar_probe_enable(isar_context* ctx, int stage) {

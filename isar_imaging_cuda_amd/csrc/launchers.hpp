@@ -71,6 +71,7 @@ void launch_accept_image(int frames, int n, const float* img_a, const double* en
 
 // standalone batched FFT used by the tests: n-point transforms of `batch` contiguous rows
 bool debug_fft(int n, int dir, const float2* in, float2* out, int batch, hipStream_t s);
+bool debug_fft_grouped(int n, int dir, const float2* in, float2* out, int batch, hipStream_t s);
 
 // stage.hip: intermediate arrays of a finished run, rebuilt for inspection
 void launch_stage_hpc(const float2* echo, const double* vel, double hpc_k, int M, int N, float2* out, hipStream_t s);

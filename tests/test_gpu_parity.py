@@ -24,6 +24,19 @@ def test_device_fft_matches_numpy(n, inverse):
     assert err < 2e-6, f"n={n} inverse={inverse} rel err {err:.2e}"
 
 
+@pytest.mark.parametrize("n", [64, 128, 192, 256, 384, 512, 1024])
+@pytest.mark.parametrize("inverse", [False, True])
+def test_wave_ordered_fft_equals_workgroup_ordered(n, inverse):
+    """Transforms several per workgroup, each within one wave and ordered by that wave alone (as
+    the autofocus, chirp-z and MTRC kernels run them), give the one-transform-per-workgroup
+    results bit for bit; a batch that leaves the last workgroup partly empty, and a single row."""
+    rng = np.random.default_rng(100 + n + int(inverse))
+    for rows in (1, 37):
+        x = (rng.standard_normal((rows, n)) + 1j * rng.standard_normal((rows, n))).astype(np.complex64)
+        print(f"[case] n={n} inverse={inverse} rows={rows}", flush=True)
+        assert np.array_equal(isar.fft(x, inverse=inverse, grouped=True), isar.fft(x, inverse=inverse))
+
+
 def _frame(config_id, **over):
     cfg = synth.preset(config_id, **over)
     echo, vel = synth.frame(cfg)
