@@ -27,6 +27,87 @@ __device__ void bitonic_sort_f(float* a, int n) {
         }
     }
 }
+
+// The same compare-exchange network (stage (k, j) orders each pair i < i ^ j ascending where
+// i & k == 0, descending otherwise, swapping when (a[i] > a[i ^ j]) == ascending), with the stages
+// inside a wave taken out of LDS: thread t holds elements t E ... t E + E - 1 in registers, stages
+// j < E exchange within the thread, stages E <= j < 64 E with the partner thread t ^ (j / E) of
+// the same wave by shuffles, and only the stages j >= 64 E go through LDS with a barrier each
+// (10 of the 105 stages at 16384 elements). Every pair meets the same comparator in the same
+// stage as in bitonic_sort_f, so the sorted array is the same bit for bit, NaNs included.
+// n = NT E; every thread of the workgroup calls it, with a in LDS and a barrier after its fill.
+template <int NT, int E>
+__device__ void bitonic_sort_f_waves(float* a) {
+    constexpr int n = NT * E, WE = 64 * E;
+    const int t = threadIdx.x, i0 = t * E;
+    float v[E];
+    bool in_regs = false;
+    for (int k = 2; k <= n; k <<= 1) {
+        int j = k >> 1;
+        for (; j >= WE; j >>= 1) {   // partner in another wave
+            if (in_regs) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) a[i0 + e] = v[e];
+                __syncthreads();
+                in_regs = false;
+            }
+            for (int i = t; i < n; i += NT) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const float x = a[i], y = a[l];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) { a[i] = y; a[l] = x; }
+                }
+            }
+            __syncthreads();
+        }
+        if (!in_regs) {   // the last LDS stage (or the caller's fill) ended with a barrier
+#pragma unroll
+            for (int e = 0; e < E; ++e) v[e] = a[i0 + e];
+            in_regs = true;
+        }
+        for (; j >= E; j >>= 1) {   // partner thread t ^ (j / E) of the same wave
+            const bool lower = (i0 & j) == 0;
+            const bool up = (i0 & k) == 0;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const float p = __shfl_xor(v[e], j / E, 64);
+                // x = the lower index's value, y = the higher one's
+                const float x = lower ? v[e] : p, y = lower ? p : v[e];
+                const bool sw = (x > y) == up;
+                v[e] = sw ? p : v[e];
+            }
+        }
+#pragma unroll
+        for (int jj = E / 2; jj >= 1; jj >>= 1) {   // within the thread (jj < k)
+            if (jj < k) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    if ((e & jj) == 0) {
+                        const float x = v[e], y = v[e | jj];
+                        const bool up = ((i0 + e) & k) == 0;
+                        if ((x > y) == up) { v[e] = y; v[e | jj] = x; }
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) a[i0 + e] = v[e];
+    __syncthreads();
+}
+
+template <int NT>
+__device__ void sort_arp(float* a, int n) {
+    switch (n) {
+    case 1024: bitonic_sort_f_waves<NT, 1024 / NT>(a); break;
+    case 2048: bitonic_sort_f_waves<NT, 2048 / NT>(a); break;
+    case 4096: bitonic_sort_f_waves<NT, 4096 / NT>(a); break;
+    case 8192: bitonic_sort_f_waves<NT, 8192 / NT>(a); break;
+    case 16384: bitonic_sort_f_waves<NT, 16384 / NT>(a); break;
+    default: bitonic_sort_f<NT>(a, n); break;
+    }
+}
 }  // namespace
 
 // Rows of arp_part / max_part holding partials, in summation order: row 0 (rows 0-1) and, at
@@ -96,7 +177,7 @@ __global__ __launch_bounds__(kCenterThreads) void k_center(Job J) {
         srt[n] = v;
     }
     __syncthreads();
-    bitonic_sort_f<NT>(srt, N);
+    sort_arp<NT>(srt, N);
 
     // threshold from the sorted ARP (range_alignment.cu:402-423)
     const float amax = fabsf(srt[N - 1]);
