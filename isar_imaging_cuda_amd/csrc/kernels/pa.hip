@@ -24,24 +24,55 @@ __device__ void doppler_phase(const float2* r, int M, float2* phi, double* scrat
     __syncthreads();
 }
 
-// ascending bitonic sort of (key, tag) pairs; ties ordered by tag (stable for unique tags)
+// Ascending bitonic sort of NT (key, tag) pairs, one per thread, ties ordered by tag (stable for
+// unique tags), NaN keys last: stage (k, j) orders each pair i < i ^ j ascending where i & k == 0,
+// descending otherwise. The stages whose partner lies in the same wave (j < 64) exchange through
+// shuffles in registers; only those across waves go through LDS, with a barrier each (6 of the 45
+// stages at 512). key and tag are in LDS, filled before a barrier; every thread calls it.
 template <int NT>
-__device__ void bitonic_sort_kv(float* key, int* tag, int n) {
-    for (int k = 2; k <= n; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < n; i += NT) {
-                const int l = i ^ j;
-                if (l > i) {
-                    const float a = key[i], b = key[l];
-                    const int ta = tag[i], tb = tag[l];
-                    const bool gt = (a > b) || (a == b && ta > tb) || (a != a && b == b);
-                    const bool up = (i & k) == 0;
-                    if (gt == up) { key[i] = b; key[l] = a; tag[i] = tb; tag[l] = ta; }
-                }
+__device__ void bitonic_sort_kv(float* key, int* tag) {
+    static_assert(NT >= 64 && (NT & (NT - 1)) == 0, "one wave or more, a power of two");
+    const int t = threadIdx.x;
+    auto greater = [](float a, float b, int ta, int tb) { return (a > b) || (a == b && ta > tb) || (a != a && b == b); };
+    float kv = 0.0f;
+    int tv = 0;
+    bool in_regs = false;
+    for (int k = 2; k <= NT; k <<= 1) {
+        int j = k >> 1;
+        for (; j >= 64; j >>= 1) {   // partner in another wave
+            if (in_regs) {
+                key[t] = kv;
+                tag[t] = tv;
+                __syncthreads();
+                in_regs = false;
+            }
+            const int l = t ^ j;
+            if (l > t) {
+                const float a = key[t], b = key[l];
+                const int ta = tag[t], tb = tag[l];
+                if (greater(a, b, ta, tb) == ((t & k) == 0)) { key[t] = b; key[l] = a; tag[t] = tb; tag[l] = ta; }
             }
             __syncthreads();
         }
+        if (!in_regs) {   // after the last LDS stage's barrier (or the caller's)
+            kv = key[t];
+            tv = tag[t];
+            in_regs = true;
+        }
+        for (; j >= 1; j >>= 1) {   // partner lane t ^ j of the same wave
+            const float pk = __shfl_xor(kv, j, 64);
+            const int pt = __shfl_xor(tv, j, 64);
+            const bool lower = (t & j) == 0;
+            const bool gt = lower ? greater(kv, pk, tv, pt) : greater(pk, kv, pt, tv);
+            if (gt == ((t & k) == 0)) {
+                kv = pk;
+                tv = pt;
+            }
+        }
     }
+    key[t] = kv;   // the last stage (j = 1) left the pairs in registers
+    tag[t] = tv;
+    __syncthreads();
 }
 
 }  // namespace
@@ -144,14 +175,14 @@ __global__ __launch_bounds__(W) void k_pa_select(Job J) {
     key[n] = (E == E) ? -E : __int_as_float(0x7f800000);
     tag[n] = n;
     __syncthreads();
-    bitonic_sort_kv<W>(key, tag, W);
+    bitonic_sort_kv<W>(key, tag);
     order1[n] = tag[n];
     __syncthreads();
     // sort 2: entropy ascending among the first tgt, rank ascending on ties
     key[n] = n < tgt ? J.colH[static_cast<size_t>(f) * W + order1[n]] : __int_as_float(0x7f800000);
     tag[n] = n < tgt ? n : W + n;
     __syncthreads();
-    bitonic_sort_kv<W>(key, tag, W);
+    bitonic_sort_kv<W>(key, tag);
     // first U of them (bin 0 fills positions beyond tgt, as std::vector growth does)
     int bin = 0x7fffffff;
     if (n < U) bin = (n < tgt) ? order1[tag[n]] : 0;
@@ -159,7 +190,7 @@ __global__ __launch_bounds__(W) void k_pa_select(Job J) {
     key[n] = static_cast<float>(bin == 0x7fffffff ? 1 << 30 : bin);
     tag[n] = bin;
     __syncthreads();
-    bitonic_sort_kv<W>(key, tag, W);
+    bitonic_sort_kv<W>(key, tag);
     if (n < U) J.sel[static_cast<size_t>(f) * W + n] = tag[n];
     if (n == 0) {
         J.dec[f].tgt_num = tgt;
