@@ -228,7 +228,15 @@ __device__ __forceinline__ float2 af_term(float2 s, float2 x, float invM) {
     else return cscale(cconjmul(s, x), invM);
 }
 
-template <int M>
+// The scale exponent of a frame's selected bins: 2^-e takes the largest selected |S| into [1, 2)
+// (e = 0 for no finite positive maximum). cm: the workgroup's maximum of their column maxima.
+__device__ __forceinline__ int af_scale_exponent(float cm) {
+    return cm > 0.0f && cm <= 3.40282347e38f ? ilogbf(cm) : 0;
+}
+
+// PRE = true: the selected bins are in su and their Doppler products in rowdop already
+// (k_af_gather, k_af_bin_doppler), and only the start phase is formed (no iterations).
+template <int M, bool PRE = false>
 __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     using A = AfCfg<M>;
     using C = ColCfg<M>;
@@ -254,10 +262,11 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     // the largest selected |S| lies in [1, 2): the update takes the unit phasor of sums cubic in
     // S, and a power of two scales every product exactly, so the phases are those of the unscaled
     // bins while conj(S) IFFT(G |G|) stays far from FP32 overflow at any input level
+    if constexpr (!PRE) {
     float cm = 0.0f;
     for (int u = tid; u < U; u += NT) cm = fmaxf(cm, J.colmax[static_cast<size_t>(f) * W + sel[u]]);
     cm = block_max<NT>(cm, reinterpret_cast<float*>(red));
-    const int e = cm > 0.0f && cm <= 3.40282347e38f ? ilogbf(cm) : 0;
+    const int e = af_scale_exponent(cm);
     {   // the selected bins' indices in LDS (the transform buffers are free here), then the
         // gather with eight loads in flight per thread
         int* lsel = reinterpret_cast<int*>(bufs);
@@ -279,9 +288,12 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
         }
     }
     __syncthreads();
+    }
     if (J.phi0_given) {
         for (int m = tid; m < M; m += NT) phi[m] = J.phi0_in[static_cast<size_t>(f) * M + m];
         __syncthreads();
+    } else if constexpr (PRE) {
+        doppler_phase<NT>(J.rowdop + static_cast<size_t>(f) * M, M, phi, scr, red);
     } else {
         // Doppler tracking of the selected bins without compensation (:282), the products added
         // in bin order with eight bins' loads in flight
@@ -306,6 +318,10 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
         doppler_phase<NT>(rdop, M, phi, scr, red);
     }
     for (int m = tid; m < M; m += NT) J.phi0[static_cast<size_t>(f) * M + m] = phi[m];
+    if constexpr (PRE) {
+        for (int m = tid; m < M; m += NT) J.phi[static_cast<size_t>(f) * M + m] = phi[m];
+        return;
+    }
 
     const int g = tid / TPF, lane = tid % TPF;
     float2* buf = bufs + g * lds_padded_size(M);
@@ -385,6 +401,71 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
         __syncthreads();
     }
     for (int m = tid; m < M; m += NT) J.phi[static_cast<size_t>(f) * M + m] = phi[m];
+}
+
+// Start of the split and resident forms with the frame's work spread over many workgroups (a
+// batch of a few frames leaves most CUs idle under one workgroup per frame): k_af_gather copies
+// the selected bins into su as k_autofocus does (same scale exponent, same values), kAfGatherGroups
+// workgroups per frame each taking a contiguous share of the (pulse, bin) elements; then
+// k_af_bin_doppler forms each pulse's Doppler product of the selected bins (:282) with the
+// products added in bin order as k_autofocus adds them, into rowdop, which is free once the bins
+// are selected; k_autofocus<M, true> then forms the start phase from it.
+constexpr int kAfGatherGroups = 32;
+
+template <int M>
+__global__ __launch_bounds__(256) void k_af_gather(Job J) {
+    HIP_DYNAMIC_SHARED(int, lsel);                                 // W
+    float* red = reinterpret_cast<float*>(lsel + J.W);             // 32
+    const int f = blockIdx.y, tid = threadIdx.x;
+    const int W = J.W;
+    const int U = J.dec[f].unit2_num;
+    const int* sel = J.sel + static_cast<size_t>(f) * W;
+    float cm = 0.0f;
+    for (int u = tid; u < U; u += 256) cm = fmaxf(cm, J.colmax[static_cast<size_t>(f) * W + sel[u]]);
+    cm = block_max<256>(cm, red);
+    const int e = af_scale_exponent(cm);
+    for (int u = tid; u < U; u += 256) lsel[u] = sel[u];
+    __syncthreads();
+    const float2* X = J.x + static_cast<size_t>(f) * M * W;
+    float2* su = J.su + static_cast<size_t>(f) * W * M;
+    const int total = U * M, share = (total + gridDim.x - 1) / gridDim.x;
+    const int i0 = blockIdx.x * share, i1 = min(i0 + share, total);
+    for (int idx0 = i0 + tid; idx0 < i1; idx0 += 8 * 256) {
+        float2 v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int idx = idx0 + i * 256;
+            if (idx < i1) v[i] = X[static_cast<size_t>(idx / U) * W + lsel[idx % U]];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int idx = idx0 + i * 256;
+            if (idx < i1) su[static_cast<size_t>(idx % U) * M + idx / U] = make_float2(ldexpf(v[i].x, -e), ldexpf(v[i].y, -e));
+        }
+    }
+}
+
+template <int M>
+__global__ __launch_bounds__(256) void k_af_bin_doppler(Job J) {
+    const int f = blockIdx.y, m = blockIdx.x * 256 + threadIdx.x;
+    if (m >= M - 1) return;
+    const int U = J.dec[f].unit2_num;
+    const float2* su = J.su + static_cast<size_t>(f) * J.W * M;
+    float2 acc = make_float2(0.0f, 0.0f);
+    for (int u0 = 0; u0 < U; u0 += 8) {
+        float2 a[8], b[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            if (u0 + i < U) {
+                a[i] = su[static_cast<size_t>(u0 + i) * M + m];
+                b[i] = su[static_cast<size_t>(u0 + i) * M + m + 1];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (u0 + i < U) acc = cadd(acc, cconjmul(a[i], b[i]));
+    }
+    J.rowdop[static_cast<size_t>(f) * M + m] = acc;
 }
 
 // The same fixed-point update with a frame spread over several workgroups, for batches too small
@@ -690,6 +771,7 @@ hipError_t configure_pa_kernels() {
 #define ISAR_SET_PA(MM)                                                          \
     set(reinterpret_cast<const void*>(&k_pa_colfft<MM>), colfft_lds<MM>());      \
     set(reinterpret_cast<const void*>(&k_autofocus<MM>), autofocus_lds<MM>());   \
+    set(reinterpret_cast<const void*>(&k_autofocus<MM, true>), autofocus_lds<MM>());   \
     set(reinterpret_cast<const void*>(&k_af_cols<MM, true>), af_cols_lds<MM>(true));   \
     set(reinterpret_cast<const void*>(&k_af_cols<MM, false>), af_cols_lds<MM>(false));   \
     set(reinterpret_cast<const void*>(&k_af_resident<MM>), af_cols_lds<MM>(true));
@@ -712,11 +794,20 @@ void launch_autofocus(const Job& J, hipStream_t s) {
     ISAR_PULSE_SWITCH(J.M, ISAR_LAUNCH(k_autofocus<MM>, dim3(J.K), dim3(AfCfg<MM>::NT), autofocus_lds<MM>(), s, J));
 }
 namespace {
+// The start phase and the gathered bins of the split and resident forms (k_af_gather,
+// k_af_bin_doppler, k_autofocus<MM, true>).
 template <int MM>
-void autofocus_split(const Job& J, bool fold, hipStream_t s) {
+void autofocus_start_spread(const Job& J, hipStream_t s) {
+    ISAR_LAUNCH(k_af_gather<MM>, dim3(kAfGatherGroups, J.K), dim3(256), J.W * sizeof(int) + 32 * sizeof(float), s, J);
+    if (!J.phi0_given) ISAR_LAUNCH(k_af_bin_doppler<MM>, dim3((MM - 1 + 255) / 256, J.K), dim3(256), 0, s, J);
     Job J0 = J;
     J0.iterations = 0;
-    ISAR_LAUNCH(k_autofocus<MM>, dim3(J.K), dim3(AfCfg<MM>::NT), autofocus_lds<MM>(), s, J0);
+    ISAR_LAUNCH((k_autofocus<MM, true>), dim3(J.K), dim3(AfCfg<MM>::NT), autofocus_lds<MM>(), s, J0);
+}
+
+template <int MM>
+void autofocus_split(const Job& J, bool fold, hipStream_t s) {
+    autofocus_start_spread<MM>(J, s);
     constexpr int SUB = AfCfg<MM>::SUB;
     const dim3 cols_grid(J.W / SUB, J.K);
     const dim3 cols_block(SUB * AfCfg<MM>::TPF);
@@ -740,9 +831,7 @@ void launch_autofocus_split(const Job& J, bool fold, hipStream_t s) {
 namespace {
 template <int MM>
 void autofocus_resident(const Job& J, hipStream_t s) {
-    Job J0 = J;
-    J0.iterations = 0;
-    ISAR_LAUNCH(k_autofocus<MM>, dim3(J.K), dim3(AfCfg<MM>::NT), autofocus_lds<MM>(), s, J0);
+    autofocus_start_spread<MM>(J, s);
     if (J.iterations > 0) {
         constexpr int SUB = AfCfg<MM>::SUB;
         ISAR_LAUNCH_RESIDENT(k_af_resident<MM>, dim3(J.W / SUB, J.K), dim3(SUB * AfCfg<MM>::TPF), af_cols_lds<MM>(true), s,
