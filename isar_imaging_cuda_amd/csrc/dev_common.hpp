@@ -603,6 +603,8 @@ struct FftPassesCached {
     __device__ __forceinline__ static void run(float2 (&x)[E], float2* lds, int tid, const FftTwiddles<N, E>& tw,
                                                const float2* tab = nullptr) {
         fft_check_wave<N, E, WAVE>();
+        // the LDS table is filled by the whole workgroup and ordered by the first workgroup barrier
+        static_assert(!(TAB && WAVE), "table twiddles need workgroup-ordered passes");
         constexpr int R = Plan::R[P];
         if constexpr (E == R) {   // one radix group per thread: twiddles from the cache
             if constexpr (TAB && Ns == 16 && R == 16) {
