@@ -263,31 +263,31 @@ __global__ __launch_bounds__(AfCfg<M>::NT) void k_autofocus(Job J) {
     // S, and a power of two scales every product exactly, so the phases are those of the unscaled
     // bins while conj(S) IFFT(G |G|) stays far from FP32 overflow at any input level
     if constexpr (!PRE) {
-    float cm = 0.0f;
-    for (int u = tid; u < U; u += NT) cm = fmaxf(cm, J.colmax[static_cast<size_t>(f) * W + sel[u]]);
-    cm = block_max<NT>(cm, reinterpret_cast<float*>(red));
-    const int e = af_scale_exponent(cm);
-    {   // the selected bins' indices in LDS (the transform buffers are free here), then the
-        // gather with eight loads in flight per thread
-        int* lsel = reinterpret_cast<int*>(bufs);
-        for (int u = tid; u < U; u += NT) lsel[u] = sel[u];
-        __syncthreads();
-        for (int idx0 = tid; idx0 < U * M; idx0 += 8 * NT) {
-            float2 v[8];
+        float cm = 0.0f;
+        for (int u = tid; u < U; u += NT) cm = fmaxf(cm, J.colmax[static_cast<size_t>(f) * W + sel[u]]);
+        cm = block_max<NT>(cm, reinterpret_cast<float*>(red));
+        const int e = af_scale_exponent(cm);
+        {   // the selected bins' indices in LDS (the transform buffers are free here), then the
+            // gather with eight loads in flight per thread
+            int* lsel = reinterpret_cast<int*>(bufs);
+            for (int u = tid; u < U; u += NT) lsel[u] = sel[u];
+            __syncthreads();
+            for (int idx0 = tid; idx0 < U * M; idx0 += 8 * NT) {
+                float2 v[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int idx = idx0 + i * NT;
-                if (idx < U * M) v[i] = X[static_cast<size_t>(idx / U) * W + lsel[idx % U]];
-            }
+                for (int i = 0; i < 8; ++i) {
+                    const int idx = idx0 + i * NT;
+                    if (idx < U * M) v[i] = X[static_cast<size_t>(idx / U) * W + lsel[idx % U]];
+                }
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int idx = idx0 + i * NT;
-                if (idx < U * M)
-                    su[static_cast<size_t>(idx % U) * M + idx / U] = make_float2(ldexpf(v[i].x, -e), ldexpf(v[i].y, -e));
+                for (int i = 0; i < 8; ++i) {
+                    const int idx = idx0 + i * NT;
+                    if (idx < U * M)
+                        su[static_cast<size_t>(idx % U) * M + idx / U] = make_float2(ldexpf(v[i].x, -e), ldexpf(v[i].y, -e));
+                }
             }
         }
-    }
-    __syncthreads();
+        __syncthreads();
     }
     if (J.phi0_given) {
         for (int m = tid; m < M; m += NT) phi[m] = J.phi0_in[static_cast<size_t>(f) * M + m];
